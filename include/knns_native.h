@@ -1,0 +1,107 @@
+/* Native C ABI of the MI355X brute-force kNN engine.
+ *
+ * The drop-in interface (cuknns.h) mirrors the reference. This header is the engine's own
+ * interface: device-resident corpora, int32 indices, precision modes, batches with per-query k,
+ * all-kNN self exclusion and error codes. It has no counterpart in the reference; the drop-in
+ * entry points of cuknns.h are thin wrappers over it. Every function is synchronous on return
+ * unless stated otherwise, and thread-safe (calls are serialised per process).
+ */
+#ifndef KNNS_NATIVE_H
+#define KNNS_NATIVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  KNNS_OK = 0,
+  KNNS_ERR_INVALID = 1,      /* bad argument (shape, k, dtype, precision/corpus mismatch) */
+  KNNS_ERR_NOMEM = 2,        /* device or host allocation failed, or over the memory budget */
+  KNNS_ERR_DEVICE = 3,       /* HIP runtime error or no device */
+  KNNS_ERR_UNSUPPORTED = 4,  /* valid request outside what this build implements */
+  KNNS_ERR_OVERFLOW = 5      /* candidate pool overflow (massive exact ties); results invalid */
+} knns_status;
+
+typedef enum { KNNS_F32 = 0, KNNS_F64 = 1 } knns_dtype;
+
+typedef enum {
+  KNNS_PREC_FP32 = 0,  /* FP32 MFMA shortlist, exact rerank (the drop-in form) */
+  KNNS_PREC_BF16 = 1,  /* BF16 MFMA shortlist of `shortlist` candidates, exact rerank */
+  KNNS_PREC_FP8 = 2,   /* FP8 e4m3 (per-row scale) shortlist, exact rerank */
+  KNNS_PREC_FP64 = 3   /* FP64 corpus/queries: FP32 MFMA shortlist, FP64 rerank + output */
+} knns_precision;
+
+typedef enum {
+  KNNS_SELECT_AUTO = 0,
+  KNNS_SELECT_THRESHOLD = 1, /* heap entry point: threshold filter + insertion, radix-select merge */
+  KNNS_SELECT_BITONIC = 2    /* bitonic entry point: truncated bitonic merge of the shortlist */
+} knns_select;
+
+/* corpus flags */
+#define KNNS_CORPUS_DEVICE_PTR 0x1u /* `data` is device memory */
+#define KNNS_CORPUS_BF16 0x2u       /* also build the BF16 operand copy */
+#define KNNS_CORPUS_FP8 0x4u        /* also build the FP8 operand copy */
+
+/* search flags */
+#define KNNS_DEVICE_PTRS 0x1u   /* queries and outputs are device memory */
+#define KNNS_EXCLUDE_SELF 0x2u  /* query i never returns corpus row self_offset + i */
+#define KNNS_DIST_F64 0x4u      /* write double distances (default: float, or double for FP64) */
+
+typedef struct knns_corpus_s* knns_corpus;
+
+typedef struct {
+  int precision;     /* knns_precision */
+  int select;        /* knns_select */
+  int k;             /* neighbours per query (ignored by knns_search_multi) */
+  int shortlist;     /* candidates kept before the rerank; 0 = precision default */
+  unsigned flags;    /* KNNS_DEVICE_PTRS | KNNS_EXCLUDE_SELF | KNNS_DIST_F64 */
+  long self_offset;  /* with KNNS_EXCLUDE_SELF: query i is corpus row self_offset + i */
+  int ratio;         /* corpus growth factor between threshold rounds; 0 = planner */
+  int slices;        /* corpus slices per query; 0 = planner */
+  void* stream;      /* hipStream_t to run on; NULL = the engine's stream */
+} knns_params;
+
+typedef struct {
+  double total_s;    /* whole call, device time */
+  double scan_s;     /* fused distance + filter kernels */
+  double select_s;   /* merge + final (rerank) kernels */
+  long rounds;
+  long slow_compactions;
+  long candidates;   /* total entries appended by the scan kernels */
+  int groups, slices_max, qb, dc, dp;
+} knns_stats;
+
+/* fills *p with the defaults of a precision */
+void knns_params_init(knns_params* p, int precision, int k);
+
+knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, long m, int d,
+                               unsigned flags);
+knns_status knns_corpus_destroy(knns_corpus c);
+long knns_corpus_rows(knns_corpus c);
+int knns_corpus_dim(knns_corpus c);
+
+/* n queries (row-major n x d of `dtype`), k = p->k. dist: n*k (float, or double for FP64 or
+ * KNNS_DIST_F64), idx: n*k int32, row i = neighbours of query i sorted by (distance, index). */
+knns_status knns_search(knns_corpus c, const void* queries, int dtype, long n,
+                        const knns_params* p, void* dist, int32_t* idx);
+
+/* Packed batch with per-query k: query i writes kq[i] entries at offset sum_{j<i} kq[j]. */
+knns_status knns_search_multi(knns_corpus c, const void* queries, int dtype, long n,
+                              const int* kq, const knns_params* p, void* dist, int32_t* idx);
+
+/* Planner view: the launch configuration knns_search would use (no GPU work). */
+knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns_stats* out);
+
+const char* knns_last_error(void);
+knns_status knns_last_stats(knns_stats* out);
+int knns_device_count(void);
+/* releases the cached workspace of the engine */
+void knns_release_workspace(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KNNS_NATIVE_H */

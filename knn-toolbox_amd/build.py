@@ -1,0 +1,67 @@
+"""Build recipe for the in-tree HIP libraries (gfx950).
+
+    python knn-toolbox_amd/build.py            # builds lib/libknns_amd.so, libcuknn.so, libcuknn_double.so
+
+Outputs go to knn-toolbox_amd/lib/ (git-ignored, shipped with the tree). Every HIP source is built
+with -packed-fp32-ops disabled on the device side; the host pass warns that it does not know the
+feature, which is harmless.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+LIB = PKG / "lib"
+INC = PKG.parent / "include"
+ARCH = os.environ.get("KNNS_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", str(INC), "-Wall",
+          "-Wno-unused-function", "-Wno-unused-variable"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    if out and os.environ.get("KNNS_BUILD_VERBOSE"):
+        print(out, file=sys.stderr)
+
+
+def build(verbose: bool = False) -> list[Path]:
+    LIB.mkdir(exist_ok=True)
+    objs = []
+    jobs = []
+    for src in ["knns_kernels.hip", "knns_engine.hip"]:
+        obj = LIB / (Path(src).stem + ".o")
+        jobs.append([HIPCC, *COMMON, *DEVICE_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(obj)
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        list(ex.map(_run, jobs))
+    core = LIB / "libknns_amd.so"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(core)])
+    outs = [core]
+    for name, extra in [("libcuknn.so", []), ("libcuknn_double.so", ["-D__DOUBLE__"])]:
+        so = LIB / name
+        _run([HIPCC, *COMMON, *DEVICE_FLAGS, *extra, "-shared", str(CSRC / "cuknns_dropin.cpp"),
+              "-L", str(LIB), "-lknns_amd", "-Wl,-rpath,$ORIGIN", "-o", str(so)])
+        outs.append(so)
+    for o in objs:
+        o.unlink(missing_ok=True)
+    if verbose:
+        for o in outs:
+            print(f"built {o}")
+    return outs
+
+
+if __name__ == "__main__":
+    build(verbose=True)

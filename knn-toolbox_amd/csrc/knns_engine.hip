@@ -1,0 +1,626 @@
+// Host engine and native C ABI (include/knns_native.h).
+//
+// One search = pack queries -> R threshold rounds (scan kernel over a growing corpus prefix,
+// then merge) -> final rerank. The rounds replace the reference's corpus chunk x stream loop and
+// host merge (src/cuknnBitonic.cu:550-589, src/cuknnHeap.cu:509-548): instead of merging P
+// partial top-k lists on the host, every round starts from the exact K-th candidate of all the
+// corpus rows seen so far, so the filter inside the scan kernel stays tight.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/knns_native.h"
+#include "knns_internal.h"
+
+struct knns_corpus_s {
+  long m = 0;
+  int d = 0;
+  int dp = 0, dc = 0;
+  long rows_pad = 0;
+  long ld32 = 0, ld16 = 0;
+  int dtype = KNNS_F32;
+  unsigned flags = 0;
+  float* x32 = nullptr;
+  uint16_t* x16 = nullptr;
+  double* x64 = nullptr;
+};
+
+namespace knns {
+namespace {
+
+thread_local std::string g_err;
+
+knns_status fail(knns_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+#define KNNS_HIP(expr)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(KNNS_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                      \
+  } while (0)
+
+enum Slot {
+  W_Q32, W_Q16, W_Q64, W_STAGE, W_THR, W_KC, W_KQ, W_OFF, W_SELF, W_SUB, W_SUBCNT, W_CARRY0,
+  W_CARRY1, W_CCNT0, W_CCNT1, W_FLAGS, W_ODIST, W_OIDX, W_NSLOTS
+};
+
+struct Engine {
+  std::recursive_mutex mu;
+  bool ready = false;
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  void* buf[W_NSLOTS] = {};
+  size_t cap[W_NSLOTS] = {};
+  std::vector<hipEvent_t> events;
+  knns_stats last{};
+
+  knns_status init() {
+    if (ready) return KNNS_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+      return fail(KNNS_ERR_DEVICE, "no HIP device available");
+    KNNS_HIP(hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    KNNS_HIP(hipGetDeviceProperties(&prop, device));
+    cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    KNNS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    ready = true;
+    return KNNS_OK;
+  }
+
+  knns_status get(Slot s, size_t bytes, void** out) {
+    if (bytes == 0) bytes = 256;
+    if (cap[s] < bytes) {
+      if (buf[s]) (void)hipFree(buf[s]);
+      buf[s] = nullptr;
+      cap[s] = 0;
+      const size_t want = bytes + bytes / 8;
+      if (hipMalloc(&buf[s], want) != hipSuccess) {
+        (void)hipGetLastError();
+        buf[s] = nullptr;
+        return fail(KNNS_ERR_NOMEM, "device allocation of %zu bytes failed", want);
+      }
+      cap[s] = want;
+    }
+    *out = buf[s];
+    return KNNS_OK;
+  }
+
+  void release() {
+    for (int s = 0; s < W_NSLOTS; ++s) {
+      if (buf[s]) (void)hipFree(buf[s]);
+      buf[s] = nullptr;
+      cap[s] = 0;
+    }
+  }
+
+  hipEvent_t event(size_t i) {
+    while (events.size() <= i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      events.push_back(e);
+    }
+    return events[i];
+  }
+};
+
+Engine& engine() {
+  static Engine* e = new Engine();  // never destroyed: HIP may already be torn down at exit
+  return *e;
+}
+
+inline long round_up(long x, long a) { return (x + a - 1) / a * a; }
+
+// padded dimension and MFMA chunk for a dimension d
+void dims_for(int d, int* dp, int* dc) {
+  const int d16 = (int)round_up(d, 16);
+  if (d16 <= 32) *dc = 32;
+  else if (d16 <= 64) *dc = 64;
+  else if (d16 <= 96) *dc = 96;
+  else *dc = 128;
+  *dp = d16 <= 128 ? *dc : (int)round_up(d, 128);
+}
+
+int default_shortlist(int prec, int k) {
+  switch (prec) {
+    case KNNS_PREC_BF16: return 2 * k + 32;
+    case KNNS_PREC_FP8: return 4 * k + 32;
+    default: return k + 16;
+  }
+}
+
+struct Round {
+  long begin, end, slice_rows;
+  int slices;
+};
+
+struct Plan {
+  int prec_kernel, qb, dc, dp, groups, n_pad, slices_max, sub_cap, kmax, cmax;
+  std::vector<Round> rounds;
+};
+
+Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
+               int slices_req) {
+  Plan P{};
+  dims_for(d, &P.dp, &P.dc);
+  P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : PREC_F32;
+  P.qb = (P.prec_kernel == PREC_BF16 && n > 32) ? 2 : 1;
+  const int qw = 32 * P.qb;
+  P.groups = (int)((n + qw - 1) / qw);
+  P.n_pad = P.groups * qw;
+  P.kmax = kmax;
+  P.cmax = cmax;
+  const long target_waves = (long)cus * 4 * 2;
+  int smax = (int)std::min<long>(128, std::max<long>(1, (target_waves + P.groups - 1) / P.groups));
+  if (slices_req > 0) smax = std::min(128, slices_req);
+  const int ratio = ratio_req >= 2 ? ratio_req : 2;
+  long b = 0;
+  long n0 = std::min<long>(m, round_up(std::max<long>(2L * cmax, 64), 64));
+  long e = n0;
+  int smax_used = 1;
+  while (b < m) {
+    Round r;
+    r.begin = b;
+    r.end = e;
+    const long rows = e - b;
+    int s = (int)std::min<long>(smax, std::max<long>(1, rows / 256));
+    r.slice_rows = round_up((rows + s - 1) / s, 64);
+    r.slices = (int)((rows + r.slice_rows - 1) / r.slice_rows);
+    smax_used = std::max(smax_used, r.slices);
+    P.rounds.push_back(r);
+    b = e;
+    e = std::min<long>(m, round_up(e * ratio, 64));
+  }
+  P.slices_max = smax_used;
+  P.sub_cap = (int)round_up(std::max<long>(cmax + 96, 256), 64);
+  return P;
+}
+
+bool is_device_ptr(const void* p) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice;
+}
+
+knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, const int* kq_host,
+                      int kuni, const knns_params* p, void* dist, int32_t* idx) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lock(E.mu);
+  g_err.clear();
+  if (knns_status st = E.init()) return st;
+  if (!c) return fail(KNNS_ERR_INVALID, "null corpus");
+  if (!p) return fail(KNNS_ERR_INVALID, "null params");
+  if (n < 0) return fail(KNNS_ERR_INVALID, "negative query count %ld", n);
+  if (n == 0) return KNNS_OK;
+  if (!queries || !dist || !idx) return fail(KNNS_ERR_INVALID, "null query/output pointer");
+  if (dtype != c->dtype)
+    return fail(KNNS_ERR_INVALID, "query dtype %d does not match corpus dtype %d", dtype, c->dtype);
+  if (n > (1L << 30)) return fail(KNNS_ERR_INVALID, "too many queries (%ld)", n);
+  const int prec = p->precision;
+  if (prec == KNNS_PREC_FP8)
+    return fail(KNNS_ERR_UNSUPPORTED, "FP8 shortlist is not available in this build");
+  if (prec != KNNS_PREC_FP32 && prec != KNNS_PREC_BF16 && prec != KNNS_PREC_FP64)
+    return fail(KNNS_ERR_INVALID, "unknown precision %d", prec);
+  if (prec == KNNS_PREC_FP64 && c->dtype != KNNS_F64)
+    return fail(KNNS_ERR_INVALID, "FP64 precision needs a double corpus");
+  if (prec == KNNS_PREC_BF16 && !c->x16)
+    return fail(KNNS_ERR_INVALID, "BF16 precision needs a corpus built with KNNS_CORPUS_BF16");
+  const bool dev = (p->flags & KNNS_DEVICE_PTRS) != 0;
+  const bool excl = (p->flags & KNNS_EXCLUDE_SELF) != 0;
+  const bool out64 = c->dtype == KNNS_F64 || (p->flags & KNNS_DIST_F64);
+  hipStream_t st = p->stream ? (hipStream_t)p->stream : E.stream;
+  const long m = c->m;
+
+  // per-query k and shortlist
+  std::vector<int> kq(n), kc(n);
+  std::vector<long> off(n);
+  long total_out = 0;
+  int kmax = 0, cmax = 0;
+  const long m_eff = excl ? std::max<long>(m - 1, 1) : m;
+  for (long i = 0; i < n; ++i) {
+    const int k = kq_host ? kq_host[i] : kuni;
+    if (k < 1 || k > 1024) return fail(KNNS_ERR_INVALID, "k=%d outside [1, 1024]", k);
+    int cl = p->shortlist > 0 ? std::max(p->shortlist, k) : default_shortlist(prec, k);
+    cl = (int)std::min<long>(cl, m_eff);
+    cl = std::min(cl, 2048);
+    kq[i] = k;
+    kc[i] = std::max(cl, 1);
+    off[i] = total_out;
+    total_out += k;
+    kmax = std::max(kmax, k);
+    cmax = std::max(cmax, kc[i]);
+  }
+
+  Plan P = make_plan(m, c->d, n, kmax, cmax, prec, E.cus, p->ratio, p->slices);
+  if (P.dp != c->dp) return fail(KNNS_ERR_INVALID, "corpus/plan dimension mismatch");
+  const long n_pad = P.n_pad;
+
+  // workspace
+  void *q32, *q16 = nullptr, *q64 = nullptr, *thr, *kcd, *kqd, *offd, *selfd = nullptr, *sub,
+             *subcnt, *car0, *car1, *cc0, *cc1, *flags, *odist, *oidx;
+  knns_status sst;
+#define KNNS_WS(slot, bytes, ptr) \
+  if ((sst = E.get(slot, (size_t)(bytes), &ptr)) != KNNS_OK) return sst
+  KNNS_WS(W_Q32, n_pad * c->ld32 * 4, q32);
+  if (P.prec_kernel == PREC_BF16) KNNS_WS(W_Q16, n_pad * c->ld16 * 2, q16);
+  if (c->dtype == KNNS_F64) KNNS_WS(W_Q64, n * (long)c->d * 8, q64);
+  KNNS_WS(W_THR, n_pad * 4, thr);
+  KNNS_WS(W_KC, n_pad * 4, kcd);
+  KNNS_WS(W_KQ, n * 4, kqd);
+  KNNS_WS(W_OFF, n * 8, offd);
+  if (excl) KNNS_WS(W_SELF, n_pad * 4, selfd);
+  KNNS_WS(W_SUB, (size_t)n_pad * P.slices_max * 2 * P.sub_cap * 8, sub);
+  KNNS_WS(W_SUBCNT, (size_t)n_pad * P.slices_max * 2 * 4, subcnt);
+  KNNS_WS(W_CARRY0, (size_t)n * P.cmax * 8, car0);
+  KNNS_WS(W_CARRY1, (size_t)n * P.cmax * 8, car1);
+  KNNS_WS(W_CCNT0, n * 4, cc0);
+  KNNS_WS(W_CCNT1, n * 4, cc1);
+  KNNS_WS(W_FLAGS, 16 * 4, flags);
+  if (!dev) {
+    KNNS_WS(W_ODIST, total_out * (out64 ? 8 : 4), odist);
+    KNNS_WS(W_OIDX, total_out * 4, oidx);
+  } else {
+    odist = dist;
+    oidx = idx;
+  }
+#undef KNNS_WS
+
+  size_t ev = 0;
+  hipEvent_t e_begin = E.event(ev++);
+  if (!e_begin) return fail(KNNS_ERR_DEVICE, "event creation failed");
+  KNNS_HIP(hipEventRecord(e_begin, st));
+
+  // queries -> packed operand rows
+  const size_t qsz = dtype == KNNS_F64 ? 8 : 4;
+  const void* qsrc = queries;
+  if (!dev) {
+    void* stage;
+    if ((sst = E.get(W_STAGE, n * (long)c->d * qsz, &stage)) != KNNS_OK) return sst;
+    KNNS_HIP(hipMemcpyAsync(stage, queries, n * (long)c->d * qsz, hipMemcpyHostToDevice, st));
+    qsrc = stage;
+  }
+  PackArgs pa{};
+  pa.src = qsrc;
+  pa.src_f64 = dtype == KNNS_F64;
+  pa.rows = n;
+  pa.rows_pad = n_pad;
+  pa.d = c->d;
+  pa.ld_src = c->d;
+  pa.x32 = (float*)q32;
+  pa.ld32 = c->ld32;
+  pa.x16 = (uint16_t*)q16;
+  pa.ld16 = c->ld16;
+  pa.x64 = (double*)q64;
+  pa.dp = c->dp;
+  pa.is_query = 1;
+  KNNS_HIP(launch_pack(pa, st));
+
+  // per-query control arrays
+  {
+    std::vector<uint32_t> thr_h(n_pad, ORD_NEG_INF);
+    for (long i = 0; i < n; ++i) thr_h[i] = ORD_POS_INF;
+    std::vector<int> kc_h(n_pad, 1);
+    for (long i = 0; i < n; ++i) kc_h[i] = kc[i];
+    KNNS_HIP(hipMemcpyAsync(thr, thr_h.data(), n_pad * 4, hipMemcpyHostToDevice, st));
+    KNNS_HIP(hipMemcpyAsync(kcd, kc_h.data(), n_pad * 4, hipMemcpyHostToDevice, st));
+    KNNS_HIP(hipMemcpyAsync(kqd, kq.data(), n * 4, hipMemcpyHostToDevice, st));
+    KNNS_HIP(hipMemcpyAsync(offd, off.data(), n * 8, hipMemcpyHostToDevice, st));
+    if (excl) {
+      std::vector<int> self_h(n_pad, -1);
+      for (long i = 0; i < n; ++i) {
+        const long r = p->self_offset + i;
+        self_h[i] = (r >= 0 && r < m) ? (int)r : -1;
+      }
+      KNNS_HIP(hipMemcpyAsync(selfd, self_h.data(), n_pad * 4, hipMemcpyHostToDevice, st));
+    }
+    KNNS_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
+  }
+  KNNS_HIP(launch_fill_i32((int*)cc0, 0, n, st));
+  KNNS_HIP(launch_fill_i32((int*)flags, 0, 16, st));
+
+  const void* xop = P.prec_kernel == PREC_BF16 ? (const void*)c->x16 : (const void*)c->x32;
+  const void* qop = P.prec_kernel == PREC_BF16 ? q16 : q32;
+  const long ldx = P.prec_kernel == PREC_BF16 ? c->ld16 : c->ld32;
+  const int bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
+
+  std::vector<std::pair<size_t, size_t>> scan_ev, sel_ev;
+  void* carry[2] = {car0, car1};
+  void* ccnt[2] = {cc0, cc1};
+  int cur = 0;
+  for (size_t r = 0; r < P.rounds.size(); ++r) {
+    const Round& R = P.rounds[r];
+    ScanArgs sa{};
+    sa.X = xop;
+    sa.Qp = qop;
+    sa.ldx = ldx;
+    sa.ldq = ldx;
+    sa.dp = P.dp;
+    sa.row_begin = R.begin;
+    sa.row_end = R.end;
+    sa.slice_rows = R.slice_rows;
+    sa.slices = R.slices;
+    sa.slices_max = P.slices_max;
+    sa.n_pad = (int)n_pad;
+    sa.thr = (const uint32_t*)thr;
+    sa.kc = (const int*)kcd;
+    sa.sub = (uint64_t*)sub;
+    sa.sub_cnt = (int*)subcnt;
+    sa.sub_cap = P.sub_cap;
+    sa.self_ids = (const int*)selfd;
+    sa.flags = (int*)flags;
+    const size_t e0 = ev++, e1 = ev++;
+    KNNS_HIP(hipEventRecord(E.event(e0), st));
+    KNNS_HIP(launch_scan(P.prec_kernel, P.qb, P.dc, sa, P.groups, st));
+    KNNS_HIP(hipEventRecord(E.event(e1), st));
+    scan_ev.push_back({e0, e1});
+
+    MergeArgs ma{};
+    ma.carry_in = (const uint64_t*)carry[cur];
+    ma.carry_in_cnt = (const int*)ccnt[cur];
+    ma.carry_out = (uint64_t*)carry[cur ^ 1];
+    ma.carry_out_cnt = (int*)ccnt[cur ^ 1];
+    ma.carry_cap = P.cmax;
+    ma.sub = (const uint64_t*)sub;
+    ma.sub_cnt = (const int*)subcnt;
+    ma.sub_cap = P.sub_cap;
+    ma.slices = R.slices;
+    ma.slices_max = P.slices_max;
+    ma.thr = (uint32_t*)thr;
+    ma.kc = (const int*)kcd;
+    ma.n = (int)n;
+    ma.bitonic = bitonic;
+    ma.flags = (int*)flags;
+    const size_t e2 = ev++;
+    KNNS_HIP(launch_merge(ma, st));
+    KNNS_HIP(hipEventRecord(E.event(e2), st));
+    sel_ev.push_back({e1, e2});
+    cur ^= 1;
+  }
+
+  FinalArgs fa{};
+  fa.carry = (const uint64_t*)carry[cur];
+  fa.carry_cnt = (const int*)ccnt[cur];
+  fa.carry_cap = P.cmax;
+  fa.kc = (const int*)kcd;
+  fa.kq = (const int*)kqd;
+  fa.out_off = (const long*)offd;
+  if (c->dtype == KNNS_F64) {
+    fa.Xm = c->x64; fa.ldxm = c->d; fa.Qm = q64; fa.ldqm = c->d; fa.master_f64 = 1;
+  } else {
+    fa.Xm = c->x32; fa.ldxm = c->ld32; fa.Qm = q32; fa.ldqm = c->ld32; fa.master_f64 = 0;
+  }
+  fa.d = c->d;
+  fa.out_f64 = out64 ? 1 : 0;
+  fa.out_dist = odist;
+  fa.out_idx = (int32_t*)oidx;
+  fa.n = (int)n;
+  const size_t e_pre = ev++;
+  KNNS_HIP(hipEventRecord(E.event(e_pre), st));
+  const size_t e3 = ev++;
+  KNNS_HIP(launch_final(fa, st));
+  KNNS_HIP(hipEventRecord(E.event(e3), st));
+  sel_ev.push_back({e_pre, e3});
+
+  int flags_h[16] = {0};
+  KNNS_HIP(hipMemcpyAsync(flags_h, flags, sizeof(flags_h), hipMemcpyDeviceToHost, st));
+  if (!dev) {
+    KNNS_HIP(hipMemcpyAsync(dist, odist, total_out * (out64 ? 8 : 4), hipMemcpyDeviceToHost, st));
+    KNNS_HIP(hipMemcpyAsync(idx, oidx, total_out * 4, hipMemcpyDeviceToHost, st));
+  }
+  const size_t e_end = ev++;
+  KNNS_HIP(hipEventRecord(E.event(e_end), st));
+  KNNS_HIP(hipStreamSynchronize(st));
+
+  knns_stats S{};
+  float ms = 0.f;
+  KNNS_HIP(hipEventElapsedTime(&ms, e_begin, E.event(e_end)));
+  S.total_s = ms * 1e-3;
+  for (auto& pr : scan_ev) {
+    KNNS_HIP(hipEventElapsedTime(&ms, E.event(pr.first), E.event(pr.second)));
+    S.scan_s += ms * 1e-3;
+  }
+  for (auto& pr : sel_ev) {
+    KNNS_HIP(hipEventElapsedTime(&ms, E.event(pr.first), E.event(pr.second)));
+    S.select_s += ms * 1e-3;
+  }
+  S.rounds = (long)P.rounds.size();
+  S.slow_compactions = flags_h[1];
+  S.groups = P.groups;
+  S.slices_max = P.slices_max;
+  S.qb = P.qb;
+  S.dc = P.dc;
+  S.dp = P.dp;
+  E.last = S;
+  if (flags_h[0] != 0)
+    return fail(KNNS_ERR_OVERFLOW, "candidate pool overflow (%d events): too many exact ties",
+                flags_h[0]);
+  return KNNS_OK;
+}
+
+}  // namespace
+}  // namespace knns
+
+using namespace knns;
+
+extern "C" {
+
+void knns_params_init(knns_params* p, int precision, int k) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->precision = precision;
+  p->select = KNNS_SELECT_AUTO;
+  p->k = k;
+}
+
+knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, long m, int d,
+                               unsigned flags) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lock(E.mu);
+  g_err.clear();
+  if (!out) return fail(KNNS_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  if (!data) return fail(KNNS_ERR_INVALID, "null corpus data");
+  if (m < 1 || m > 0x7fffff00L) return fail(KNNS_ERR_INVALID, "corpus rows %ld out of range", m);
+  if (d < 1 || d > 65536) return fail(KNNS_ERR_INVALID, "dimension %d out of range", d);
+  if (dtype != KNNS_F32 && dtype != KNNS_F64) return fail(KNNS_ERR_INVALID, "bad dtype %d", dtype);
+  if (flags & KNNS_CORPUS_FP8)
+    return fail(KNNS_ERR_UNSUPPORTED, "FP8 corpus copy is not available in this build");
+  if (knns_status st = E.init()) return st;
+  knns_corpus c = new knns_corpus_s();
+  c->m = m;
+  c->d = d;
+  c->dtype = dtype;
+  c->flags = flags;
+  dims_for(d, &c->dp, &c->dc);
+  c->rows_pad = round_up(m, 64);
+  c->ld32 = c->dp + 4;
+  c->ld16 = c->dp + 16;
+  auto cleanup = [&](knns_status st) {
+    if (c->x32) (void)hipFree(c->x32);
+    if (c->x16) (void)hipFree(c->x16);
+    if (c->x64) (void)hipFree(c->x64);
+    delete c;
+    (void)hipGetLastError();
+    return st;
+  };
+  if (hipMalloc((void**)&c->x32, (size_t)c->rows_pad * c->ld32 * 4) != hipSuccess)
+    return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP32 rows (%ld x %ld)", c->rows_pad, c->ld32));
+  if ((flags & KNNS_CORPUS_BF16) &&
+      hipMalloc((void**)&c->x16, (size_t)c->rows_pad * c->ld16 * 2) != hipSuccess)
+    return cleanup(fail(KNNS_ERR_NOMEM, "corpus BF16 rows"));
+  if (dtype == KNNS_F64 && hipMalloc((void**)&c->x64, (size_t)m * d * 8) != hipSuccess)
+    return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP64 master"));
+  const size_t esz = dtype == KNNS_F64 ? 8 : 4;
+  const bool dev = (flags & KNNS_CORPUS_DEVICE_PTR) != 0;
+  const long chunk = dev ? m : std::min<long>(m, 1L << 20);
+  void* stage = nullptr;
+  if (!dev && hipMalloc(&stage, (size_t)chunk * d * esz) != hipSuccess)
+    return cleanup(fail(KNNS_ERR_NOMEM, "corpus staging buffer"));
+  hipStream_t st = E.stream;
+  for (long r0 = 0; r0 < c->rows_pad; r0 += chunk) {
+    const long rows_w = std::min(chunk, c->rows_pad - r0);
+    const long rows = std::max<long>(0, std::min(rows_w, m - r0));
+    const void* src = (const char*)data + (size_t)r0 * d * esz;
+    if (!dev && rows > 0) {
+      if (hipMemcpyAsync(stage, src, (size_t)rows * d * esz, hipMemcpyHostToDevice, st) != hipSuccess) {
+        (void)hipFree(stage);
+        return cleanup(fail(KNNS_ERR_DEVICE, "corpus upload failed"));
+      }
+      src = stage;
+    }
+    PackArgs pa{};
+    pa.src = src;
+    pa.src_f64 = dtype == KNNS_F64;
+    pa.rows = rows;
+    pa.rows_pad = rows_w;
+    pa.d = d;
+    pa.ld_src = d;
+    pa.x32 = c->x32 + (size_t)r0 * c->ld32;
+    pa.ld32 = c->ld32;
+    pa.x16 = c->x16 ? c->x16 + (size_t)r0 * c->ld16 : nullptr;
+    pa.ld16 = c->ld16;
+    pa.x64 = c->x64 ? c->x64 + (size_t)r0 * d : nullptr;
+    pa.dp = c->dp;
+    pa.is_query = 0;
+    if (launch_pack(pa, st) != hipSuccess) {
+      if (stage) (void)hipFree(stage);
+      return cleanup(fail(KNNS_ERR_DEVICE, "corpus pack launch failed"));
+    }
+    if (!dev && hipStreamSynchronize(st) != hipSuccess) {
+      (void)hipFree(stage);
+      return cleanup(fail(KNNS_ERR_DEVICE, "corpus pack failed"));
+    }
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    if (stage) (void)hipFree(stage);
+    return cleanup(fail(KNNS_ERR_DEVICE, "corpus pack failed"));
+  }
+  if (stage) (void)hipFree(stage);
+  *out = c;
+  return KNNS_OK;
+}
+
+knns_status knns_corpus_destroy(knns_corpus c) {
+  if (!c) return KNNS_OK;
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lock(E.mu);
+  if (c->x32) (void)hipFree(c->x32);
+  if (c->x16) (void)hipFree(c->x16);
+  if (c->x64) (void)hipFree(c->x64);
+  delete c;
+  return KNNS_OK;
+}
+
+long knns_corpus_rows(knns_corpus c) { return c ? c->m : 0; }
+int knns_corpus_dim(knns_corpus c) { return c ? c->d : 0; }
+
+knns_status knns_search(knns_corpus c, const void* queries, int dtype, long n,
+                        const knns_params* p, void* dist, int32_t* idx) {
+  if (!p) return fail(KNNS_ERR_INVALID, "null params");
+  return do_search(c, queries, dtype, n, nullptr, p->k, p, dist, idx);
+}
+
+knns_status knns_search_multi(knns_corpus c, const void* queries, int dtype, long n,
+                              const int* kq, const knns_params* p, void* dist, int32_t* idx) {
+  if (!kq && n > 0) return fail(KNNS_ERR_INVALID, "null per-query k array");
+  return do_search(c, queries, dtype, n, kq, 0, p, dist, idx);
+}
+
+knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns_stats* out) {
+  if (!out || m < 1 || d < 1 || n < 1 || k < 1 || k > 1024)
+    return fail(KNNS_ERR_INVALID, "bad plan arguments");
+  const int cl = std::min<long>(std::min(default_shortlist(precision, k), 2048), m);
+  Plan P = make_plan(m, d, n, k, cl, precision, 256, 0, 0);
+  std::memset(out, 0, sizeof(*out));
+  out->rounds = (long)P.rounds.size();
+  out->groups = P.groups;
+  out->slices_max = P.slices_max;
+  out->qb = P.qb;
+  out->dc = P.dc;
+  out->dp = P.dp;
+  return KNNS_OK;
+}
+
+const char* knns_last_error(void) { return g_err.c_str(); }
+
+knns_status knns_last_stats(knns_stats* out) {
+  if (!out) return KNNS_ERR_INVALID;
+  *out = engine().last;
+  return KNNS_OK;
+}
+
+int knns_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+void knns_release_workspace(void) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lock(E.mu);
+  E.release();
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Internal declarations shared by the HIP kernels and the host engine (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace knns {
+
+// Precision of the distance GEMM in the scan kernel.
+enum Prec : int { PREC_F32 = 0, PREC_BF16 = 1, PREC_FP8 = 2 };
+
+// Candidate entries are 64-bit keys: high word = order-preserving image of the float key
+// s = ||r||^2/2 - q.r, low word = corpus row. Unsigned comparison of two entries is the
+// lexicographic (key, index) order, which is the tie rule of demo/knn.m:5 (stable sort).
+__host__ __device__ inline uint32_t f2ord(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float ord2f(uint32_t v) {
+  uint32_t u = (v & 0x80000000u) ? (v & 0x7fffffffu) : ~v;
+  return __builtin_bit_cast(float, u);
+}
+constexpr uint32_t ORD_POS_INF = 0xff800000u;  // f2ord(+inf)
+constexpr uint32_t ORD_NEG_INF = 0x007fffffu;  // f2ord(-inf)
+
+// Packed row layouts (elements per row). Main dims are padded with zeros to DP; the augmentation
+// block carries -||r||^2/2 for corpus rows and ones for query rows so that the MFMA accumulates
+// q.r - ||r||^2/2 directly.
+//   FP32: [DP main][hi, lo, 0, 0]           lane half h reads aug element h
+//   BF16: [DP main][8 aug for half 0][8 zeros for half 1]; corpus aug = 3-term bf16 split
+inline int aug_f32() { return 4; }
+inline int aug_bf16() { return 16; }
+
+struct ScanArgs {
+  const void* X;         // packed corpus rows
+  const void* Qp;        // packed query rows
+  long ldx, ldq;         // row strides in elements
+  int dp;                // padded main dims (multiple of the chunk)
+  long row_begin, row_end;  // this round's corpus range
+  long slice_rows;       // rows per slice (multiple of 64)
+  int slices;            // slices (domains per query) in this round
+  int slices_max;        // stride of the sub-pool array
+  int n_pad;             // padded query count
+  const uint32_t* thr;   // per query ordered threshold (strict): admit f2ord(s) < thr
+  const int* kc;         // per query shortlist size
+  uint64_t* sub;         // sub-pools [q][slices_max][2][sub_cap]
+  int* sub_cnt;          // [q][slices_max][2]
+  int sub_cap;
+  const int* self_ids;   // optional: corpus row excluded for each query (all-kNN), -1 = none
+  int* flags;            // [0] = overflow counter, [1] = slow-path compactions
+};
+
+struct MergeArgs {
+  const uint64_t* carry_in;  // [q][carry_cap]
+  const int* carry_in_cnt;
+  uint64_t* carry_out;
+  int* carry_out_cnt;
+  int carry_cap;
+  const uint64_t* sub;
+  const int* sub_cnt;
+  int sub_cap;
+  int slices;            // slices used by the round just scanned
+  int slices_max;
+  uint32_t* thr;         // updated per query
+  const int* kc;
+  int n;                 // real queries
+  int bitonic;           // 1: exact truncated sort (keeps exactly K); 0: histogram bound
+  int* flags;
+};
+
+struct FinalArgs {
+  const uint64_t* carry;
+  const int* carry_cnt;
+  int carry_cap;
+  const int* kc;         // shortlist size per query
+  const int* kq;         // k per query
+  const long* out_off;   // output offset (in entries) per query
+  const void* Xm;        // rerank master corpus (float or double rows)
+  long ldxm;
+  const void* Qm;        // rerank master queries
+  long ldqm;
+  int d;
+  int master_f64;        // 1: master rows are double
+  int out_f64;           // 1: write double distances
+  void* out_dist;
+  int32_t* out_idx;
+  int n;
+};
+
+struct PackArgs {
+  const void* src;       // rows of float or double
+  int src_f64;
+  long rows;             // real rows
+  long rows_pad;         // rows written (padding rows are zero / neutral)
+  int d;
+  long ld_src;
+  float* x32;            // [rows_pad][ld32]
+  long ld32;
+  uint16_t* x16;         // optional bf16 rows [rows_pad][ld16]
+  long ld16;
+  double* x64;           // optional double master copy [rows][d]
+  int dp;                // padded dims
+  int is_query;          // aug = ones instead of -||r||^2/2
+};
+
+// Launchers (knns_kernels.hip). All enqueue on `stream` and return hipError_t.
+hipError_t launch_pack(const PackArgs& a, hipStream_t stream);
+hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, hipStream_t stream);
+hipError_t launch_merge(const MergeArgs& a, hipStream_t stream);
+hipError_t launch_final(const FinalArgs& a, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t stream);
+hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t stream);
+hipError_t launch_to_knntype(const float* dsrc_f32, const double* dsrc_f64, const int32_t* isrc,
+                             void* dist_out, void* idx_out, int out_f64, long count,
+                             hipStream_t stream);
+bool scan_supported(int prec, int qb, int dc);
+
+}  // namespace knns

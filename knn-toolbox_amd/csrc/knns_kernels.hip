@@ -1,0 +1,759 @@
+// HIP kernels for the brute-force kNN path on CDNA4 (gfx950).
+//
+//   pack_kernel   rows -> padded FP32 / BF16 operand rows with the norm augmentation, FP64 master
+//   scan_kernel   fused MFMA distance + per-query threshold filter + append (one wave per
+//                 (query group, corpus slice)); replaces the GEMM + norm kernels + selection of the
+//                 reference (include/utils.h:883-985, src/cuknnBitonic.cu:45-254,
+//                 src/cuknnHeap.cu:146-291) without ever writing the n x m key matrix
+//   merge_kernel  per query: union of the round's candidates and the carried shortlist, exact
+//                 K-th entry by radix select (threshold mode) or truncated bitonic sort (bitonic
+//                 mode), new strict threshold for the next round
+//   final_kernel  per query: sort the shortlist, exact rerank with the canonical FP64 distance,
+//                 order by (distance, index), write the k nearest
+//
+// Keys. The scan ranks s = ||r~||^2/2 - q~.r~ where q~, r~ are the operands as rounded for the
+// MFMA precision and ||r~||^2 is the exact norm of the rounded row. The reference ranks
+// ||r||^2 - 2 q.r (include/utils.h:957-985), i.e. 2s. Using the norm of the rounded row makes s
+// exactly (||q~ - r~||^2 - ||q~||^2)/2, so the shortlist error scales with ||q - r||, not ||q||.
+#include "knns_internal.h"
+
+namespace knns {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even (finite inputs)
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint64_t w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint64_t w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t w = __shfl_up(v, o, 64);
+    if (lane >= o) v += w;
+  }
+  return v;
+}
+__device__ __forceinline__ int bits64(uint64_t x) { return x ? 64 - __clzll(x) : 0; }
+
+// ------------------------------------------------------------------------------------------------
+// block-level K-th smallest of N distinct 64-bit entries (radix select, 8 bits per level).
+// Works for any blockDim multiple of 64. get(i) returns entry i. Requires 1 <= K <= N.
+// lds: >= 256 + 16 uint32 of shared scratch.
+// ------------------------------------------------------------------------------------------------
+template <class Get>
+__device__ uint64_t block_kth(Get get, int N, int K, uint32_t* lds) {
+  uint32_t* hist = lds;                          // 256 bins
+  uint64_t* red = (uint64_t*)(lds + 256);        // 2 x (waves) u64, waves <= 4
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  uint64_t mn = ~0ull, mx = 0;
+  for (int i = tid; i < N; i += nt) {
+    uint64_t e = get(i);
+    mn = e < mn ? e : mn;
+    mx = e > mx ? e : mx;
+  }
+  mn = wave_min_u64(mn);
+  mx = wave_max_u64(mx);
+  if (lane == 0) { red[wv] = mn; red[4 + wv] = mx; }
+  __syncthreads();
+  mn = red[0]; mx = red[4];
+  for (int w = 1; w < nw; ++w) {
+    mn = red[w] < mn ? red[w] : mn;
+    mx = red[4 + w] > mx ? red[4 + w] : mx;
+  }
+  __syncthreads();
+  if (K >= N) return mx;
+  if (K <= 1) return mn;
+  uint64_t base = mn;
+  int below = 0;
+  int sh = bits64(mx - mn);
+  sh = sh > 8 ? sh - 8 : 0;
+  uint64_t result = mx;
+  for (int level = 0; level < 9; ++level) {
+    for (int b = tid; b < 256; b += nt) hist[b] = 0;
+    __syncthreads();
+    for (int i = tid; i < N; i += nt) {
+      uint64_t e = get(i);
+      if (e >= base) {
+        uint64_t r = (e - base) >> sh;
+        if (r < 256) atomicAdd(&hist[(int)r], 1u);
+      }
+    }
+    __syncthreads();
+    // wave 0 finds the bin holding the K-th entry
+    if (wv == 0) {
+      uint32_t c0 = hist[lane * 4 + 0], c1 = hist[lane * 4 + 1], c2 = hist[lane * 4 + 2],
+               c3 = hist[lane * 4 + 3];
+      uint32_t tot = c0 + c1 + c2 + c3;
+      uint32_t incl = wave_incl_scan_u32(tot, lane);
+      uint32_t excl = incl - tot;
+      const uint32_t need = (uint32_t)(K - below);
+      // first lane whose inclusive count reaches `need`
+      uint64_t hit = __builtin_amdgcn_ballot_w64(incl >= need);
+      int fl = __ffsll((long long)hit) - 1;
+      if (lane == fl) {
+        uint32_t acc = excl;
+        int b = 0;
+        uint32_t cb = c0;
+        if (acc + c0 >= need) { b = 0; cb = c0; }
+        else if (acc + c0 + c1 >= need) { b = 1; acc += c0; cb = c1; }
+        else if (acc + c0 + c1 + c2 >= need) { b = 2; acc += c0 + c1; cb = c2; }
+        else { b = 3; acc += c0 + c1 + c2; cb = c3; }
+        red[0] = (uint64_t)(lane * 4 + b);
+        red[1] = (uint64_t)acc;      // entries below the chosen bin (this level)
+        red[2] = (uint64_t)cb;       // entries in the chosen bin
+      }
+    }
+    __syncthreads();
+    const int bstar = (int)red[0];
+    const int blw = (int)red[1];
+    const int cbin = (int)red[2];
+    __syncthreads();
+    below += blw;
+    base += (uint64_t)bstar << sh;
+    if (sh == 0) { result = base; break; }
+    if (below + cbin == K) {
+      // every entry of the bin is kept: the K-th entry is the bin's maximum
+      const uint64_t hi = base + ((1ull << sh) - 1);
+      uint64_t m2 = 0;
+      for (int i = tid; i < N; i += nt) {
+        uint64_t e = get(i);
+        if (e >= base && e <= hi) m2 = e > m2 ? e : m2;
+      }
+      m2 = wave_max_u64(m2);
+      if (lane == 0) red[4 + wv] = m2;
+      __syncthreads();
+      m2 = red[4];
+      for (int w = 1; w < nw; ++w) m2 = red[4 + w] > m2 ? red[4 + w] : m2;
+      __syncthreads();
+      result = m2;
+      break;
+    }
+    sh = sh > 8 ? sh - 8 : 0;
+  }
+  return result;
+}
+
+// ------------------------------------------------------------------------------------------------
+// pack: one wave per row
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
+  const int lane = threadIdx.x & 63;
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= a.rows_pad) return;
+  const bool real = row < a.rows;
+  double n32 = 0.0, n16 = 0.0;
+  for (int j = lane; j < a.dp; j += 64) {
+    double v = 0.0;
+    if (real && j < a.d) {
+      v = a.src_f64 ? ((const double*)a.src)[row * a.ld_src + j]
+                    : (double)((const float*)a.src)[row * a.ld_src + j];
+      if (a.x64) a.x64[row * a.d + j] = v;
+    }
+    const float f = (float)v;
+    a.x32[row * a.ld32 + j] = f;
+    n32 += (double)f * (double)f;
+    if (a.x16) {
+      const uint16_t b = f2bf(f);
+      a.x16[row * a.ld16 + j] = b;
+      const double bd = (double)bf2f(b);
+      n16 += bd * bd;
+    }
+  }
+  n32 = wave_sum_f64(n32);
+  n16 = wave_sum_f64(n16);
+  if (lane < 4) {
+    float v = 0.f;
+    if (a.is_query) {
+      v = (lane < 2 && real) ? 1.f : 0.f;
+    } else if (real) {
+      const double t = -0.5 * n32;
+      const float hi = (float)t;
+      v = lane == 0 ? hi : (lane == 1 ? (float)(t - (double)hi) : 0.f);
+    }
+    a.x32[row * a.ld32 + a.dp + lane] = v;
+  }
+  if (a.x16 && lane < 16) {
+    uint16_t b = 0;
+    if (real) {
+      if (a.is_query) {
+        b = lane < 3 ? f2bf(1.f) : 0;
+      } else {
+        const double t = -0.5 * n16;
+        const uint16_t t1 = f2bf((float)t);
+        const double r1 = t - (double)bf2f(t1);
+        const uint16_t t2 = f2bf((float)r1);
+        const double r2 = r1 - (double)bf2f(t2);
+        const uint16_t t3 = f2bf((float)r2);
+        b = lane == 0 ? t1 : lane == 1 ? t2 : lane == 2 ? t3 : 0;
+      }
+    }
+    a.x16[row * a.ld16 + a.dp + lane] = b;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// scan kernel
+// ------------------------------------------------------------------------------------------------
+// One wave owns QB blocks of 32 queries (MFMA columns) and one corpus slice; it streams the
+// slice in tiles of 64 rows (two 32-row MFMA blocks). MFMA 32x32 D layout: lane l holds column
+// l%32 and rows 8*(v/4) + 4*(l/32) + v%4 for v = 0..15. K is permuted so that lane half h holds
+// the contiguous dims [c*DC + h*DC/2, c*DC + (h+1)*DC/2) of chunk c, identically for A and B.
+//
+// After each tile, every (query, row) with s = -D < threshold is appended to the lane's private
+// sub-pool (query, slice, half) -- no atomics. A sub-pool close to full triggers the slow path:
+// the wave computes the exact K-th entry of the query's two sub-pools, keeps exactly K entries
+// and tightens that query's threshold.
+template <int QB>
+struct ScanState {
+  float phi[QB];        // D-space threshold: admit D > phi  (phi = -threshold)
+  uint32_t thr[QB];     // ordered threshold
+  int cnt[QB];          // this lane's sub-pool fill
+  long pbase[QB];       // this lane's sub-pool base
+  int selfid[QB];
+};
+
+template <int QB>
+__device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& st, int g, int sl,
+                                          uint32_t* lds) {
+  const int lane = threadIdx.x;
+  const int lim = a.sub_cap - 32;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const uint64_t need = __builtin_amdgcn_ballot_w64(st.cnt[qb] > lim);
+    uint32_t cols = (uint32_t)(need | (need >> 32));
+    while (cols) {
+      const int c = __ffs(cols) - 1;
+      cols &= cols - 1;
+      const int q = (g * QB + qb) * 32 + c;
+      const int n0 = __builtin_amdgcn_readlane(st.cnt[qb], c);
+      const int n1 = __builtin_amdgcn_readlane(st.cnt[qb], c + 32);
+      const int K = a.kc[q];
+      uint64_t* p0 = a.sub + (((long)q * a.slices_max + sl) * 2 + 0) * a.sub_cap;
+      uint64_t* p1 = p0 + a.sub_cap;
+      const int N = n0 + n1;
+      if (lane == 0) atomicAdd(&a.flags[1], 1);
+      if (N <= K) continue;
+      auto get = [&](int i) -> uint64_t { return i < n0 ? p0[i] : p1[i - n0]; };
+      const uint64_t E = block_kth(get, N, K, lds);
+      // in-place order-preserving compaction of the logical array [p0 | p1] (writes never
+      // overtake reads: a chunk is read completely before any of its outputs is stored)
+      int outpos = 0;
+      for (int base = 0; base < N; base += 64) {
+        const int i = base + lane;
+        uint64_t e = 0;
+        bool keep = false;
+        if (i < N) { e = get(i); keep = e <= E; }
+        const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+        const int pos = outpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32),
+                                  __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+        __builtin_amdgcn_wave_barrier();
+        if (keep) { if (pos < n0) p0[pos] = e; else p1[pos - n0] = e; }
+        outpos += __popcll(km);
+      }
+      const int k0 = outpos < n0 ? outpos : n0;
+      const int k1 = outpos - k0;
+      const uint32_t T = (uint32_t)(E >> 32);
+      if (lane == c) { st.cnt[qb] = k0; }
+      if (lane == c + 32) { st.cnt[qb] = k1; }
+      if ((lane & 31) == c) {
+        if (T < st.thr[qb]) { st.thr[qb] = T; st.phi[qb] = -ord2f(T); }
+      }
+      if (k0 > lim || k1 > lim) { if (lane == 0) atomicAdd(&a.flags[0], 1); }
+    }
+  }
+}
+
+template <int QB, bool FULL>
+__device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& st,
+                                              const f32x16 (&acc)[QB][2], long t0, long c1) {
+  const int lane = threadIdx.x;
+  const int h = lane >> 5;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float D = acc[qb][cb][v];
+        const long row = t0 + cb * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+        bool ok = D > st.phi[qb];
+        if (!FULL) ok = ok && row < c1;
+        if (ok) {
+          if ((int)row != st.selfid[qb]) {
+            const float s = -D + 0.0f;
+            const uint64_t e = ((uint64_t)f2ord(s) << 32) | (uint32_t)row;
+            if (st.cnt[qb] < a.sub_cap) {
+              a.sub[st.pbase[qb] + st.cnt[qb]] = e;
+              st.cnt[qb] += 1;
+            } else {
+              atomicAdd(&a.flags[0], 1);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int PREC, int QB, int DC>
+__global__ __launch_bounds__(64) void scan_kernel(ScanArgs a) {
+  __shared__ __attribute__((aligned(16))) uint32_t lds[256 + 32];
+  const int lane = threadIdx.x;
+  const int col = lane & 31, h = lane >> 5;
+  const int g = blockIdx.x, sl = blockIdx.y;
+  const long c0 = a.row_begin + (long)sl * a.slice_rows;
+  long c1 = c0 + a.slice_rows;
+  if (c1 > a.row_end) c1 = a.row_end;
+
+  ScanState<QB> st;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int q = (g * QB + qb) * 32 + col;
+    st.thr[qb] = a.thr[q];
+    st.phi[qb] = -ord2f(st.thr[qb]);
+    st.cnt[qb] = 0;
+    st.pbase[qb] = (((long)q * a.slices_max + sl) * 2 + h) * a.sub_cap;
+    st.selfid[qb] = a.self_ids ? a.self_ids[q] : -1;
+  }
+
+  const int nch = a.dp / DC;
+  if constexpr (PREC == PREC_F32) {
+    const float* X = (const float*)a.X;
+    const float* Q = (const float*)a.Qp;
+    constexpr int NB = DC / 2;          // B floats per lane per query block per chunk
+    float bq[QB][NB];
+    auto loadB = [&](int c) {
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        const float* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + c * DC + h * (DC / 2);
+#pragma unroll
+        for (int j = 0; j < NB; j += 4) {
+          const f32x4 t = *(const f32x4*)(qp + j);
+          bq[qb][j] = t[0]; bq[qb][j + 1] = t[1]; bq[qb][j + 2] = t[2]; bq[qb][j + 3] = t[3];
+        }
+      }
+    };
+    if (nch == 1) loadB(0);
+    for (long t0 = c0; t0 < c1; t0 += 64) {
+      f32x16 acc[QB][2];
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) acc[qb][cb] = (f32x16){0};
+      const float* xr0 = X + (t0 + col) * a.ldx;
+      const float* xr1 = xr0 + 32 * a.ldx;
+      for (int c = 0; c < nch; ++c) {
+        if (nch > 1) loadB(c);
+        const float* x0 = xr0 + c * DC + h * (DC / 2);
+        const float* x1 = xr1 + c * DC + h * (DC / 2);
+#pragma unroll
+        for (int j = 0; j < NB; j += 4) {
+          const f32x4 a0 = *(const f32x4*)(x0 + j);
+          const f32x4 a1 = *(const f32x4*)(x1 + j);
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) {
+              acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[jj], bq[qb][j + jj], acc[qb][0], 0, 0, 0);
+              acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[jj], bq[qb][j + jj], acc[qb][1], 0, 0, 0);
+            }
+          }
+        }
+      }
+      {  // augmentation: -||r||^2/2 as hi + lo
+        const float g0 = xr0[a.dp + h], g1 = xr1[a.dp + h];
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(g0, 1.0f, acc[qb][0], 0, 0, 0);
+          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(g1, 1.0f, acc[qb][1], 0, 0, 0);
+        }
+      }
+      if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
+      else scan_epilogue<QB, false>(a, st, acc, t0, c1);
+      bool trig = false;
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 32;
+      if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, lds);
+    }
+  } else {  // BF16
+    const uint16_t* X = (const uint16_t*)a.X;
+    const uint16_t* Q = (const uint16_t*)a.Qp;
+    constexpr int NS = DC / 16;         // MFMA steps per chunk
+    bf16x8 bq[QB][NS];
+    auto loadB = [&](int c) {
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        const uint16_t* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + c * DC + h * (DC / 2);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) bq[qb][j] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 8 * j));
+      }
+    };
+    if (nch == 1) loadB(0);
+    bf16x8 baug[QB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      const uint16_t* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + a.dp + 8 * h;
+      baug[qb] = __builtin_bit_cast(bf16x8, *(const u32x4*)qp);
+    }
+    for (long t0 = c0; t0 < c1; t0 += 64) {
+      f32x16 acc[QB][2];
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) acc[qb][cb] = (f32x16){0};
+      const uint16_t* xr0 = X + (t0 + col) * a.ldx;
+      const uint16_t* xr1 = xr0 + 32 * a.ldx;
+      for (int c = 0; c < nch; ++c) {
+        if (nch > 1) loadB(c);
+        const uint16_t* x0 = xr0 + c * DC + h * (DC / 2);
+        const uint16_t* x1 = xr1 + c * DC + h * (DC / 2);
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(x0 + 8 * j));
+          const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(x1 + 8 * j));
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb) {
+            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[qb][j], acc[qb][0], 0, 0, 0);
+            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[qb][j], acc[qb][1], 0, 0, 0);
+          }
+        }
+      }
+      {
+        const bf16x8 g0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(xr0 + a.dp + 8 * h));
+        const bf16x8 g1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(xr1 + a.dp + 8 * h));
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g0, baug[qb], acc[qb][0], 0, 0, 0);
+          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g1, baug[qb], acc[qb][1], 0, 0, 0);
+        }
+      }
+      if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
+      else scan_epilogue<QB, false>(a, st, acc, t0, c1);
+      bool trig = false;
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 32;
+      if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, lds);
+    }
+  }
+  // publish fills
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int q = (g * QB + qb) * 32 + col;
+    a.sub_cnt[((long)q * a.slices_max + sl) * 2 + h] = st.cnt[qb];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// merge kernel: one 256-thread block per query
+// ------------------------------------------------------------------------------------------------
+constexpr int MERGE_LDS_ENTRIES = 4096;   // 32 KB of entries gathered to LDS
+constexpr int MERGE_MAX_SRC = 1 + 2 * 128;
+
+__device__ __forceinline__ void bitonic_sort_lds_u64(uint64_t* v, int P) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int t = tid; t < P / 2; t += nt) {
+        const int pos = 2 * t - (t & (stride - 1));
+        const bool up = ((pos & size) == 0);
+        const uint64_t x = v[pos], y = v[pos + stride];
+        if ((x > y) == up) { v[pos] = y; v[pos + stride] = x; }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
+  __shared__ __attribute__((aligned(16))) uint64_t ent[MERGE_LDS_ENTRIES];
+  __shared__ int off[MERGE_MAX_SRC + 1];
+  __shared__ __attribute__((aligned(16))) uint32_t scratch[256 + 32];
+  __shared__ int outc;
+  const int q = blockIdx.x;
+  if (q >= a.n) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int nsrc = 1 + 2 * a.slices;
+  const long subq = (long)q * a.slices_max * 2;
+  // source counts -> exclusive offsets (sequential in one thread: nsrc <= 257)
+  if (tid == 0) {
+    int acc = 0;
+    off[0] = 0;
+    acc += a.carry_in_cnt[q];
+    off[1] = acc;
+    for (int s = 0; s < 2 * a.slices; ++s) {
+      acc += a.sub_cnt[subq + s];
+      off[s + 2] = acc;
+    }
+    outc = 0;
+  }
+  __syncthreads();
+  const int N = off[nsrc];
+  const int K = a.kc[q];
+  const uint64_t* cin = a.carry_in + (long)q * a.carry_cap;
+  const uint64_t* sub = a.sub + subq * a.sub_cap;
+  auto gget = [&](int i) -> uint64_t {
+    // binary search the source of logical entry i
+    int lo = 0, hi = nsrc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    const int j = i - off[lo];
+    return lo == 0 ? cin[j] : sub[(long)(lo - 1) * a.sub_cap + j];
+  };
+  uint64_t* cout = a.carry_out + (long)q * a.carry_cap;
+  if (N <= K) {
+    for (int i = tid; i < N; i += nt) cout[i] = gget(i);
+    if (tid == 0) a.carry_out_cnt[q] = N;
+    return;
+  }
+  uint64_t E;
+  if (N <= MERGE_LDS_ENTRIES) {
+    for (int i = tid; i < N; i += nt) ent[i] = gget(i);
+    if (a.bitonic) {
+      int P = 1;
+      while (P < N) P <<= 1;
+      for (int i = N + tid; i < P; i += nt) ent[i] = ~0ull;
+      bitonic_sort_lds_u64(ent, P);
+      for (int i = tid; i < K; i += nt) cout[i] = ent[i];
+      E = ent[K - 1];
+      if (tid == 0) { a.carry_out_cnt[q] = K; a.thr[q] = (uint32_t)(E >> 32); }
+      return;
+    }
+    __syncthreads();
+    auto lget = [&](int i) -> uint64_t { return ent[i]; };
+    E = block_kth(lget, N, K, scratch);
+    for (int i = tid; i < N; i += nt) {
+      const uint64_t e = ent[i];
+      if (e <= E) {
+        const int pos = atomicAdd(&outc, 1);
+        if (pos < a.carry_cap) cout[pos] = e;
+      }
+    }
+  } else {
+    E = block_kth(gget, N, K, scratch);
+    for (int i = tid; i < N; i += nt) {
+      const uint64_t e = gget(i);
+      if (e <= E) {
+        const int pos = atomicAdd(&outc, 1);
+        if (pos < a.carry_cap) cout[pos] = e;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    if (outc > a.carry_cap) outc = a.carry_cap;
+    a.carry_out_cnt[q] = outc;
+    a.thr[q] = (uint32_t)(E >> 32);
+    if (outc != K) atomicAdd(&a.flags[0], 1);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// final kernel: one 256-thread block per query
+// ------------------------------------------------------------------------------------------------
+constexpr int FINAL_MAX = 2048;
+
+__device__ __forceinline__ void bitonic_sort_lds_pair(uint64_t* k, uint32_t* v, int P) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (int t = tid; t < P / 2; t += nt) {
+        const int pos = 2 * t - (t & (stride - 1));
+        const bool up = ((pos & size) == 0);
+        const uint64_t x = k[pos], y = k[pos + stride];
+        const uint32_t xv = v[pos], yv = v[pos + stride];
+        const bool gt = (x > y) || (x == y && xv > yv);
+        if (gt == up) { k[pos] = y; k[pos + stride] = x; v[pos] = yv; v[pos + stride] = xv; }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void final_kernel(FinalArgs a) {
+  __shared__ uint64_t ek[FINAL_MAX];
+  __shared__ uint32_t ev[FINAL_MAX];
+  __shared__ double qrow[4096];
+  const int q = blockIdx.x;
+  if (q >= a.n) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  int cnt = a.carry_cnt[q];
+  if (cnt > FINAL_MAX) cnt = FINAL_MAX;
+  int K = a.kc[q];
+  const int k = a.kq[q];
+  const uint64_t* src = a.carry + (long)q * a.carry_cap;
+  int P = 1;
+  while (P < cnt) P <<= 1;
+  for (int i = tid; i < P; i += nt) ek[i] = i < cnt ? src[i] : ~0ull;
+  bitonic_sort_lds_u64(ek, P);
+  const int Kq = K < cnt ? K : cnt;
+  // query master row
+  const int d = a.d;
+  const bool dq = d <= 4096;
+  if (dq) {
+    for (int j = tid; j < d; j += nt)
+      qrow[j] = a.master_f64 ? ((const double*)a.Qm)[(long)q * a.ldqm + j]
+                             : (double)((const float*)a.Qm)[(long)q * a.ldqm + j];
+  }
+  __syncthreads();
+  // exact rerank: canonical distance, sequential FMA over the dims
+  for (int t = tid; t < Kq; t += nt) {
+    const uint32_t idx = (uint32_t)(ek[t] & 0xffffffffu);
+    double acc = 0.0;
+    if (a.master_f64) {
+      const double* r = (const double*)a.Xm + (long)idx * a.ldxm;
+      const double* qq = (const double*)a.Qm + (long)q * a.ldqm;
+      for (int j = 0; j < d; ++j) {
+        const double t1 = (dq ? qrow[j] : qq[j]) - r[j];
+        acc = __builtin_fma(t1, t1, acc);
+      }
+    } else {
+      const float* r = (const float*)a.Xm + (long)idx * a.ldxm;
+      const float* qq = (const float*)a.Qm + (long)q * a.ldqm;
+      int j = 0;
+      if ((a.ldxm & 3) == 0) {
+        for (; j + 4 <= d; j += 4) {
+          const f32x4 rv = *(const f32x4*)(r + j);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double t1 = (dq ? qrow[j + u] : (double)qq[j + u]) - (double)rv[u];
+            acc = __builtin_fma(t1, t1, acc);
+          }
+        }
+      }
+      for (; j < d; ++j) {
+        const double t1 = (dq ? qrow[j] : (double)qq[j]) - (double)r[j];
+        acc = __builtin_fma(t1, t1, acc);
+      }
+    }
+    ev[t] = idx;
+    ek[t] = __builtin_bit_cast(uint64_t, acc);   // acc >= +0: bit order == numeric order
+  }
+  int P2 = 1;
+  while (P2 < Kq) P2 <<= 1;
+  __syncthreads();
+  for (int i = Kq + tid; i < P2; i += nt) { ek[i] = ~0ull; ev[i] = ~0u; }
+  bitonic_sort_lds_pair(ek, ev, P2);
+  const long o = a.out_off[q];
+  for (int t = tid; t < k; t += nt) {
+    double dist;
+    int32_t idx;
+    if (t < Kq) { dist = __builtin_bit_cast(double, ek[t]); idx = (int32_t)ev[t]; }
+    else { dist = __builtin_inf(); idx = -1; }
+    if (a.out_f64) ((double*)a.out_dist)[o + t] = dist;
+    else ((float*)a.out_dist)[o + t] = (float)dist;
+    a.out_idx[o + t] = idx;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// fills
+// ------------------------------------------------------------------------------------------------
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_i32_kernel(int* p, int v, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------------------
+hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
+  const long blocks = (a.rows_pad + 3) / 4;
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <int PREC, int QB, int DC>
+static hipError_t scan_one(const ScanArgs& a, int groups, hipStream_t stream) {
+  hipLaunchKernelGGL((scan_kernel<PREC, QB, DC>), dim3(groups, a.slices), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+bool scan_supported(int prec, int qb, int dc) {
+  if (prec != PREC_F32 && prec != PREC_BF16) return false;
+  if (qb != 1 && qb != 2) return false;
+  if (prec == PREC_F32) return dc == 32 || dc == 64 || dc == 96 || dc == 128;
+  return dc == 32 || dc == 64 || dc == 96 || dc == 128;
+}
+
+hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, hipStream_t stream) {
+#define KNNS_SCAN_CASE(P, B, D) \
+  if (prec == P && qb == B && dc == D) return scan_one<P, B, D>(a, groups, stream);
+  KNNS_SCAN_CASE(PREC_F32, 1, 32) KNNS_SCAN_CASE(PREC_F32, 1, 64)
+  KNNS_SCAN_CASE(PREC_F32, 1, 96) KNNS_SCAN_CASE(PREC_F32, 1, 128)
+  KNNS_SCAN_CASE(PREC_F32, 2, 32) KNNS_SCAN_CASE(PREC_F32, 2, 64)
+  KNNS_SCAN_CASE(PREC_F32, 2, 96) KNNS_SCAN_CASE(PREC_F32, 2, 128)
+  KNNS_SCAN_CASE(PREC_BF16, 1, 32) KNNS_SCAN_CASE(PREC_BF16, 1, 64)
+  KNNS_SCAN_CASE(PREC_BF16, 1, 96) KNNS_SCAN_CASE(PREC_BF16, 1, 128)
+  KNNS_SCAN_CASE(PREC_BF16, 2, 32) KNNS_SCAN_CASE(PREC_BF16, 2, 64)
+  KNNS_SCAN_CASE(PREC_BF16, 2, 96) KNNS_SCAN_CASE(PREC_BF16, 2, 128)
+#undef KNNS_SCAN_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_merge(const MergeArgs& a, hipStream_t stream) {
+  if (a.n <= 0) return hipSuccess;
+  if (1 + 2 * a.slices > MERGE_MAX_SRC) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(merge_kernel, dim3(a.n), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_final(const FinalArgs& a, hipStream_t stream) {
+  if (a.n <= 0) return hipSuccess;
+  if (a.carry_cap > FINAL_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(final_kernel, dim3(a.n), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace knns

@@ -1,0 +1,281 @@
+"""Python binding of the in-tree HIP libraries (ctypes over the C ABI).
+
+Two layers, mirroring the C headers:
+
+* ``dropin`` / ``DropIn`` -- the reference's C API (include/cuknns.h): ``gpuknnsBitonic``,
+  ``gpuknnsHeap``, ``knnsplan``, ``knnsexecute``, with knntype (float32, or float64 for the
+  double library) row-major inputs and Q x k outputs holding indices as knntype values, exactly as
+  the reference's C callers see them (src/knnTest.cu:93-100).
+* ``Corpus`` -- the native engine (include/knns_native.h): resident corpus, int32 indices,
+  precision modes, per-query k, all-kNN self exclusion, device pointers.
+
+There is no CPU fallback: if the libraries are not built, or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_DIR = PKG / "lib"
+
+KNNS_OK, KNNS_ERR_INVALID, KNNS_ERR_NOMEM, KNNS_ERR_DEVICE, KNNS_ERR_UNSUPPORTED, KNNS_ERR_OVERFLOW = range(6)
+KNNS_F32, KNNS_F64 = 0, 1
+PREC = {"fp32": 0, "bf16": 1, "fp8": 2, "fp64": 3}
+SELECT = {"auto": 0, "threshold": 1, "heap": 1, "bitonic": 2}
+KNNS_CORPUS_DEVICE_PTR, KNNS_CORPUS_BF16, KNNS_CORPUS_FP8 = 0x1, 0x2, 0x4
+KNNS_DEVICE_PTRS, KNNS_EXCLUDE_SELF, KNNS_DIST_F64 = 0x1, 0x2, 0x4
+
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "OVERFLOW"}
+
+
+class KnnsError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"knns {STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("select", ctypes.c_int), ("k", ctypes.c_int),
+                ("shortlist", ctypes.c_int), ("flags", ctypes.c_uint), ("self_offset", ctypes.c_long),
+                ("ratio", ctypes.c_int), ("slices", ctypes.c_int), ("stream", ctypes.c_void_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("total_s", ctypes.c_double), ("scan_s", ctypes.c_double), ("select_s", ctypes.c_double),
+                ("rounds", ctypes.c_long), ("slow_compactions", ctypes.c_long), ("candidates", ctypes.c_long),
+                ("groups", ctypes.c_int), ("slices_max", ctypes.c_int), ("qb", ctypes.c_int),
+                ("dc", ctypes.c_int), ("dp", ctypes.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Times(ctypes.Structure):
+    _fields_ = [("dst_time", ctypes.c_double), ("srch_time", ctypes.c_double), ("knn_time", ctypes.c_double)]
+
+
+_native = None
+_dropin = {}
+
+
+def lib_path(name: str) -> Path:
+    return LIB_DIR / name
+
+
+def native():
+    """Load libknns_amd.so (raises if it was not built)."""
+    global _native
+    if _native is None:
+        p = lib_path("libknns_amd.so")
+        if not p.exists():
+            raise KnnsError(KNNS_ERR_DEVICE, f"{p} not built; run knn-toolbox_amd/build.py")
+        L = ctypes.CDLL(str(p))
+        V, I, L_, U = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_uint
+        L.knns_params_init.argtypes = [ctypes.POINTER(Params), I, I]
+        L.knns_params_init.restype = None
+        L.knns_corpus_create.argtypes = [ctypes.POINTER(V), V, I, L_, I, U]
+        L.knns_corpus_create.restype = I
+        L.knns_corpus_destroy.argtypes = [V]
+        L.knns_corpus_destroy.restype = I
+        L.knns_corpus_rows.argtypes = [V]
+        L.knns_corpus_rows.restype = L_
+        L.knns_corpus_dim.argtypes = [V]
+        L.knns_corpus_dim.restype = I
+        L.knns_search.argtypes = [V, V, I, L_, ctypes.POINTER(Params), V, V]
+        L.knns_search.restype = I
+        L.knns_search_multi.argtypes = [V, V, I, L_, V, ctypes.POINTER(Params), V, V]
+        L.knns_search_multi.restype = I
+        L.knns_plan_describe.argtypes = [L_, I, L_, I, I, ctypes.POINTER(Stats)]
+        L.knns_plan_describe.restype = I
+        L.knns_last_error.argtypes = []
+        L.knns_last_error.restype = ctypes.c_char_p
+        L.knns_last_stats.argtypes = [ctypes.POINTER(Stats)]
+        L.knns_last_stats.restype = I
+        L.knns_device_count.argtypes = []
+        L.knns_device_count.restype = I
+        L.knns_release_workspace.argtypes = []
+        L.knns_release_workspace.restype = None
+        _native = L
+    return _native
+
+
+def dropin(double: bool = False):
+    """Load libcuknn.so (float knntype) or libcuknn_double.so."""
+    if double not in _dropin:
+        p = lib_path("libcuknn_double.so" if double else "libcuknn.so")
+        if not p.exists():
+            raise KnnsError(KNNS_ERR_DEVICE, f"{p} not built; run knn-toolbox_amd/build.py")
+        L = ctypes.CDLL(str(p))
+        kt = ctypes.c_void_p
+        I = ctypes.c_int
+        for fn in ("gpuknnsBitonic", "gpuknnsHeap"):
+            f = getattr(L, fn)
+            f.argtypes = [kt, kt, kt, kt, I, I, I, I, I]
+            f.restype = ctypes.c_double
+        L.knnsGetLastError.argtypes = []
+        L.knnsGetLastError.restype = ctypes.c_char_p
+        L.knnsGetLastTimes.argtypes = []
+        L.knnsGetLastTimes.restype = Times
+        _dropin[double] = L
+    return _dropin[double]
+
+
+def _err(status: int):
+    raise KnnsError(status, native().knns_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    return native().knns_device_count()
+
+
+def last_stats() -> dict:
+    s = Stats()
+    native().knns_last_stats(ctypes.byref(s))
+    return s.as_dict()
+
+
+def plan_describe(m: int, d: int, n: int, k: int, precision: str = "fp32") -> dict:
+    s = Stats()
+    st = native().knns_plan_describe(m, d, n, k, PREC[precision], ctypes.byref(s))
+    if st:
+        _err(st)
+    return s.as_dict()
+
+
+def _ptr(a) -> int:
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())  # torch tensor
+
+
+class Corpus:
+    """A corpus resident in device memory (native API)."""
+
+    def __init__(self, data, bf16: bool = False, device_ptr: bool = False, m: int | None = None,
+                 d: int | None = None, dtype=None):
+        L = native()
+        if device_ptr:
+            self.m, self.d = int(m), int(d)
+            self.dtype = np.dtype(dtype)
+            src = _ptr(data)
+        else:
+            data = np.ascontiguousarray(data)
+            if data.dtype not in (np.float32, np.float64) or data.ndim != 2:
+                raise KnnsError(KNNS_ERR_INVALID, "corpus must be a 2-D float32/float64 array")
+            self.m, self.d = data.shape
+            self.dtype = data.dtype
+            src = data.ctypes.data
+        flags = (KNNS_CORPUS_BF16 if bf16 else 0) | (KNNS_CORPUS_DEVICE_PTR if device_ptr else 0)
+        h = ctypes.c_void_p()
+        st = L.knns_corpus_create(ctypes.byref(h), src, KNNS_F64 if self.dtype == np.float64 else KNNS_F32,
+                                  self.m, self.d, flags)
+        if st:
+            _err(st)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            native().knns_corpus_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _params(self, precision, k, select, shortlist, exclude_self, self_offset, ratio, slices, flags):
+        p = Params()
+        native().knns_params_init(ctypes.byref(p), PREC[precision], int(k))
+        p.select = SELECT[select]
+        p.shortlist = int(shortlist)
+        p.flags = flags | (KNNS_EXCLUDE_SELF if exclude_self else 0)
+        p.self_offset = int(self_offset)
+        p.ratio = int(ratio)
+        p.slices = int(slices)
+        return p
+
+    def search(self, queries, k: int, precision: str = "fp32", select: str = "auto", shortlist: int = 0,
+               exclude_self: bool = False, self_offset: int = 0, ratio: int = 0, slices: int = 0,
+               dist_f64: bool = False):
+        """Host arrays in, (dist (n,k), idx (n,k) int32) out, sorted by (distance, index)."""
+        q = np.ascontiguousarray(queries, dtype=self.dtype)
+        n = q.shape[0]
+        out64 = self.dtype == np.float64 or dist_f64
+        D = np.empty((n, k), dtype=np.float64 if out64 else np.float32)
+        I = np.empty((n, k), dtype=np.int32)
+        p = self._params(precision, k, select, shortlist, exclude_self, self_offset, ratio, slices,
+                         KNNS_DIST_F64 if dist_f64 else 0)
+        st = native().knns_search(self._h, q.ctypes.data, KNNS_F64 if self.dtype == np.float64 else KNNS_F32,
+                                  n, ctypes.byref(p), D.ctypes.data, I.ctypes.data)
+        if st:
+            _err(st)
+        return D, I
+
+    def search_device(self, q_ptr, n: int, k: int, dist_ptr, idx_ptr, precision: str = "fp32",
+                      select: str = "auto", shortlist: int = 0, ratio: int = 0, slices: int = 0,
+                      stream: int = 0):
+        """Device pointers (ints or torch tensors); synchronous."""
+        p = self._params(precision, k, select, shortlist, False, 0, ratio, slices, KNNS_DEVICE_PTRS)
+        p.stream = stream or None
+        st = native().knns_search(self._h, _ptr(q_ptr) if not isinstance(q_ptr, int) else q_ptr,
+                                  KNNS_F64 if self.dtype == np.float64 else KNNS_F32, n, ctypes.byref(p),
+                                  _ptr(dist_ptr) if not isinstance(dist_ptr, int) else dist_ptr,
+                                  _ptr(idx_ptr) if not isinstance(idx_ptr, int) else idx_ptr)
+        if st:
+            _err(st)
+
+    def search_multi(self, queries, kq, precision: str = "fp32", select: str = "auto", shortlist: int = 0):
+        """Per-query k. Returns a list of (dist, idx) per query."""
+        q = np.ascontiguousarray(queries, dtype=self.dtype)
+        kq = np.ascontiguousarray(kq, dtype=np.int32)
+        n = q.shape[0]
+        tot = int(kq.sum())
+        out64 = self.dtype == np.float64
+        D = np.empty(tot, dtype=np.float64 if out64 else np.float32)
+        I = np.empty(tot, dtype=np.int32)
+        p = self._params(precision, 0, select, shortlist, False, 0, 0, 0, 0)
+        st = native().knns_search_multi(self._h, q.ctypes.data, KNNS_F64 if out64 else KNNS_F32, n,
+                                        kq.ctypes.data, ctypes.byref(p), D.ctypes.data, I.ctypes.data)
+        if st:
+            _err(st)
+        off = np.concatenate([[0], np.cumsum(kq)])
+        return [(D[off[i]:off[i + 1]], I[off[i]:off[i + 1]]) for i in range(n)]
+
+
+# ------------------------------------------------------------------------------------------------
+# drop-in API (the reference's C interface)
+# ------------------------------------------------------------------------------------------------
+def _dropin_call(fn_name: str, query, data, k: int, double: bool, num_streams: int = 1):
+    L = dropin(double)
+    kt = np.float64 if double else np.float32
+    q = np.ascontiguousarray(query, dtype=kt)
+    x = np.ascontiguousarray(data, dtype=kt)
+    nq, d = q.shape
+    m = x.shape[0]
+    vals = np.empty((nq, k), dtype=kt)
+    inds = np.empty((nq, k), dtype=kt)
+    t = getattr(L, fn_name)(q.ctypes.data, x.ctypes.data, vals.ctypes.data, inds.ctypes.data,
+                             m, nq, d, k, num_streams)
+    if t >= float(np.finfo(np.float32).max):
+        raise KnnsError(KNNS_ERR_DEVICE, L.knnsGetLastError().decode(errors="replace"))
+    return vals, inds, t
+
+
+def gpuknns_bitonic(query, data, k: int, double: bool = False, num_streams: int = 1):
+    """gpuknnsBitonic(query, data, values, indices, objects, numQueries, attributes, k, numStreams)."""
+    return _dropin_call("gpuknnsBitonic", query, data, k, double, num_streams)
+
+
+def gpuknns_heap(query, data, k: int, double: bool = False, num_streams: int = 1):
+    return _dropin_call("gpuknnsHeap", query, data, k, double, num_streams)

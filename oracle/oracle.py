@@ -1,0 +1,409 @@
+"""CPU oracle for the brute-force kNN path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import this
+module, and only as the checker (or the timed CPU baseline). The GPU product path never does.
+
+Contents
+--------
+* ``load()``            -- the C++ restatements in ``knn_oracle.cpp`` (built by ``oracle/Makefile``):
+    - ``knnm_f32`` / ``knnm_f64``: line-for-line transcription of demo/knn.m:5-9 (the reference's CPU
+      brute force and the key formulation the GPU code ranks on, include/utils.h:957-985).
+    - ``exact_topk``: FP64 brute force with the canonical distance (sequential FMA of squared
+      differences) and (dist, index) ordering.
+* Data generators (the synthetic inputs of every workload, written down once, here).
+* Python restatements of the reference's selection kernels, run on explicit key matrices:
+    - ``ref_heap_select``   src/cuknnHeap.cu:65-174 (heap_initialize, heap_insertion),
+                            :226-250 (selection), :100-144 (final_unordered), :465 (qk)
+    - ``ref_tbis_select``   src/cuknnBitonic.cu:28-254 (Comparator, Comparator_elim, bitonic_shared,
+                            bitonic_shared2, relloc, BitonicSelect)
+    - ``ref_merge_res``     src/cuknnBitonic.cu:381-417 (mergeResBitonic host merge)
+  They pin what the reference *selects*: the k smallest keys. tests/test_oracle.py checks them
+  against a stable sort on tie-free keys and records their tie behaviour.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+FLT_MAX = np.float32(np.finfo(np.float32).max)
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile knn_oracle.cpp into oracle/_build/liboracle.so (g++ -O3 -fopenmp)."""
+    if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "knn_oracle.cpp").stat().st_mtime:
+        subprocess.run(["make", "-s", "-C", str(HERE), "lib"], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        build()
+        lib = ctypes.CDLL(str(LIB))
+        P = ctypes.c_void_p
+        L = ctypes.c_long
+        I = ctypes.c_int
+        lib.oracle_knnm_f32.argtypes = [P, P, L, L, I, I, P, P]
+        lib.oracle_knnm_f64.argtypes = [P, P, L, L, I, I, P, P]
+        lib.oracle_exact_topk.argtypes = [P, P, I, L, L, I, I, P, P, P]
+        lib.oracle_pair_dist.argtypes = [P, P, I, L, I, I, P, P]
+        lib.oracle_keys_f32.argtypes = [P, P, L, L, I, P]
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ----------------------------------------------------------------------------------------------
+# C++ restatements
+# ----------------------------------------------------------------------------------------------
+def knnm(Q: np.ndarray, R: np.ndarray, k: int):
+    """demo/knn.m:5-9 in the input precision. Returns (dist (n,k), idx (n,k) int32, 0-based)."""
+    lib = load()
+    Q = np.ascontiguousarray(Q)
+    R = np.ascontiguousarray(R)
+    n, d = Q.shape
+    m = R.shape[0]
+    assert R.shape[1] == d and Q.dtype == R.dtype and 1 <= k <= m
+    D = np.empty((n, k), dtype=Q.dtype)
+    I = np.empty((n, k), dtype=np.int32)
+    fn = lib.oracle_knnm_f32 if Q.dtype == np.float32 else lib.oracle_knnm_f64
+    fn(_p(Q), _p(R), n, m, d, k, _p(D), _p(I))
+    return D, I
+
+
+def exact_topk(Q: np.ndarray, R: np.ndarray, k: int, self_ids: np.ndarray | None = None):
+    """FP64 brute force, canonical distance, (dist, idx) order. Returns (dist f64, idx int32)."""
+    lib = load()
+    Q = np.ascontiguousarray(Q)
+    R = np.ascontiguousarray(R)
+    n, d = Q.shape
+    m = R.shape[0]
+    assert R.shape[1] == d and Q.dtype == R.dtype
+    D = np.empty((n, k), dtype=np.float64)
+    I = np.empty((n, k), dtype=np.int32)
+    sp = None
+    if self_ids is not None:
+        self_ids = np.ascontiguousarray(self_ids, dtype=np.int64)
+        sp = _p(self_ids)
+    lib.oracle_exact_topk(_p(Q), _p(R), 0 if Q.dtype == np.float32 else 1, n, m, d, k, sp, _p(D), _p(I))
+    return D, I
+
+
+def pair_dist(Q: np.ndarray, R: np.ndarray, C: np.ndarray) -> np.ndarray:
+    """Canonical FP64 distance for the explicit pairs (i, C[i, t])."""
+    lib = load()
+    Q = np.ascontiguousarray(Q)
+    R = np.ascontiguousarray(R)
+    C = np.ascontiguousarray(C, dtype=np.int32)
+    n, d = Q.shape
+    D = np.empty(C.shape, dtype=np.float64)
+    lib.oracle_pair_dist(_p(Q), _p(R), 0 if Q.dtype == np.float32 else 1, n, d, C.shape[1], _p(C), _p(D))
+    return D
+
+
+def keys_f32(Q: np.ndarray, R: np.ndarray) -> np.ndarray:
+    """The reference's key matrix fl(-2 q.r) + fl(||r||^2) (include/utils.h:957-985), shape (n, m)."""
+    lib = load()
+    Q = np.ascontiguousarray(Q, dtype=np.float32)
+    R = np.ascontiguousarray(R, dtype=np.float32)
+    K = np.empty((Q.shape[0], R.shape[0]), dtype=np.float32)
+    lib.oracle_keys_f32(_p(Q), _p(R), Q.shape[0], R.shape[0], Q.shape[1], _p(K))
+    return K
+
+
+def canonical_dist_np(q: np.ndarray, r: np.ndarray) -> float:
+    """Pure-Python canonical distance for one pair (small cases; pins the C++ one)."""
+    import math
+
+    acc = 0.0
+    for a, b in zip(q.tolist(), r.tolist()):
+        t = float(a) - float(b)
+        acc = math.fma(t, t, acc) if hasattr(math, "fma") else acc + t * t
+    return acc
+
+
+# ----------------------------------------------------------------------------------------------
+# Synthetic workloads (the generator definitions every test and the bench use)
+# ----------------------------------------------------------------------------------------------
+def uniform(n: int, d: int, seed: int, lo: float = 0.0, hi: float = 1.0, dtype=np.float32) -> np.ndarray:
+    """i.i.d. U[lo, hi) from numpy PCG64(seed), generated in float64 then rounded to dtype."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    x = g.random((n, d), dtype=np.float64)
+    return (lo + (hi - lo) * x).astype(dtype)
+
+
+def mixture_centers(ncenters: int, d: int, seed: int) -> np.ndarray:
+    g = np.random.Generator(np.random.PCG64(seed))
+    return g.standard_normal((ncenters, d))
+
+
+def mixture(n: int, d: int, ncenters: int, sigma: float, center_seed: int, point_seed: int,
+            dtype=np.float32, chunk: int = 1 << 20) -> np.ndarray:
+    """Equal-weight isotropic Gaussian mixture: centres ~ N(0, I) drawn from PCG64(center_seed);
+    each point picks a centre uniformly and adds sigma * N(0, I), both from PCG64(point_seed).
+    Queries of a workload use the corpus' centres with their own point_seed."""
+    C = mixture_centers(ncenters, d, center_seed)
+    g = np.random.Generator(np.random.PCG64(point_seed))
+    out = np.empty((n, d), dtype=dtype)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        a = g.integers(0, ncenters, size=e - s)
+        out[s:e] = (C[a] + sigma * g.standard_normal((e - s, d))).astype(dtype)
+    return out
+
+
+def log_uniform_sizes(count: int, lo: int, hi: int, seed: int, total: int | None = None) -> np.ndarray:
+    """Batch sizes log-uniform in [lo, hi]; if total is given, rescaled to sum exactly to total by
+    largest remainder."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    s = np.exp(g.uniform(np.log(lo), np.log(hi), size=count))
+    if total is None:
+        return np.maximum(1, np.round(s)).astype(np.int64)
+    w = s / s.sum() * total
+    base = np.floor(w).astype(np.int64)
+    rem = total - base.sum()
+    order = np.argsort(-(w - base), kind="stable")
+    base[order[:rem]] += 1
+    return base
+
+
+# Workload definitions (corpus/query generators). Each returns (corpus, queries).
+def workload_c1():
+    """m=16384 x d=128 U[0,1) seed 0; n=1024 U[0,1) seed 1; k=16."""
+    return uniform(16384, 128, 0), uniform(1024, 128, 1)
+
+
+def workload_c2(m: int = 1_000_000, n: int = 10_000):
+    """1024-centre mixture, sigma 0.1, d=128: corpus seed 2, queries seed 3 (same centres)."""
+    R = mixture(m, 128, 1024, 0.1, center_seed=2, point_seed=2 + 1000)
+    Q = mixture(n, 128, 1024, 0.1, center_seed=2, point_seed=3 + 1000)
+    return R, Q
+
+
+def workload_c3(m: int = 10_000_000, n: int = 10_000):
+    R = mixture(m, 96, 4096, 0.15, center_seed=4, point_seed=4 + 1000)
+    Q = mixture(n, 96, 4096, 0.15, center_seed=4, point_seed=5 + 1000)
+    return R, Q
+
+
+def workload_c4(m: int = 200_000):
+    return uniform(m, 32, 6, lo=-1.0, hi=1.0, dtype=np.float64)
+
+
+# ----------------------------------------------------------------------------------------------
+# Python restatements of the reference's selection kernels (run on explicit key rows)
+# ----------------------------------------------------------------------------------------------
+def _heap_insertion(h, idx, a, newid, qk):
+    # src/cuknnHeap.cu:146-174 (one query; Q-strided addressing collapsed to a plain list)
+    c = 0
+    h[c] = a
+    idx[c] = newid
+    for _ in range(qk - 1):
+        l = c << 1
+        r = l + 1
+        if h[l] <= h[r]:
+            if h[c] < h[r]:
+                h[c], h[r] = h[r], h[c]
+                idx[c], idx[r] = idx[r], idx[c]
+                c = r
+        else:
+            if h[c] < h[l]:
+                h[c], h[l] = h[l], h[c]
+                idx[c], idx[l] = idx[l], idx[c]
+                c = l
+
+
+def _final_unordered(data, index, N, k):
+    # src/cuknnHeap.cu:100-144 (one query)
+    mx = data[0]
+    max_ind = 0
+    for i in range(k):
+        if mx < data[i]:
+            mx = data[i]
+            max_ind = i
+    for i in range(k, N):
+        tmp = data[i]
+        if mx > tmp:
+            mx = tmp
+            data[max_ind] = tmp
+            index[max_ind] = index[i]
+            for j in range(k):
+                if mx < data[j]:
+                    mx = data[j]
+                    max_ind = j
+
+
+def ref_heap_select(keys: np.ndarray, k: int, id_offset: int = 0):
+    """Reference heap selection on one key row (float32). Returns the k live slots (key, index)
+    in heap order, exactly as the kernel leaves them (src/cuknnHeap.cu:226-250, 289)."""
+    qk = int(np.log(np.float32(k)) / np.log(2.0)) + 1  # cuknnHeap.cu:465
+    size = (1 << qk) - 1
+    h = [float(FLT_MAX)] * size                         # heap_initialize(..., FLT_MAX) :371
+    idx = [0.0] * size
+    for i, t in enumerate(np.asarray(keys, dtype=np.float32).tolist()):
+        if h[0] > t:                                    # strict admission :242
+            _heap_insertion(h, idx, t, float(i + id_offset), qk)
+    kk = 1 << (qk - 1)
+    _final_unordered(h, idx, size, kk)                  # :289
+    return np.array(h[:kk], dtype=np.float32), np.array(idx[:kk], dtype=np.float32)
+
+
+def _comparator(key, val, a, b, ddd):
+    # src/cuknnBitonic.cu:28-34, vectorised over disjoint pairs (a, b); ddd per pair
+    sw = (key[a] > key[b]) == ddd
+    ka, kb = key[a].copy(), key[b].copy()
+    va, vb = val[a].copy(), val[b].copy()
+    key[a] = np.where(sw, kb, ka)
+    key[b] = np.where(sw, ka, kb)
+    val[a] = np.where(sw, vb, va)
+    val[b] = np.where(sw, va, vb)
+
+
+def _elim_and_merge(key, val, obj_start, k, qk, dirv=1):
+    # the obj loop shared by bitonic_shared (:88-113) and bitonic_shared2 (:158-181)
+    obj = obj_start
+    while obj >= k:
+        t = np.arange(obj)
+        bi = t >> qk
+        li = t & (k - 1)
+        pb = (obj >> qk) + ((bi + 1) & ((obj >> qk) - 1))
+        prt = (pb << qk) + li
+        # Comparator_elim(s[t], s[prt], 1) (:36-43): take the partner when it is smaller
+        take = key[prt] < key[t]
+        key[t] = np.where(take, key[prt], key[t])
+        val[t] = np.where(take, val[prt], val[t])
+        size = k
+        ddd = (dirv ^ ((t & (size >> 1)) != 0).astype(np.int64)).astype(bool)
+        stride = size // 2
+        while stride > 0:
+            pos = 2 * t - (t & (stride - 1))
+            _comparator(key, val, pos, pos + stride, ddd)
+            stride >>= 1
+        obj >>= 1
+
+
+def _bitonic_shared(keys_row, k, qk, block, limit, id_offset):
+    # src/cuknnBitonic.cu:45-121 for one (block, query)
+    m = keys_row.shape[0]
+    half = limit // 2
+    t = np.arange(half)
+    key = np.empty(limit, dtype=np.float32)
+    val = np.empty(limit, dtype=np.float32)
+    g0 = block * limit + t
+    key[t] = np.where(g0 < m, keys_row[np.minimum(g0, m - 1)], FLT_MAX)
+    val[t] = (g0 + id_offset).astype(np.float32)
+    g1 = g0 + half
+    key[t + half] = np.where(g1 < m, keys_row[np.minimum(g1, m - 1)], FLT_MAX)
+    val[t + half] = (g1 + id_offset).astype(np.float32)
+    size = 2
+    while size <= k:
+        ddd = (1 ^ ((t & (size >> 1)) != 0).astype(np.int64)).astype(bool)
+        stride = size >> 1
+        while stride > 0:
+            pos = (t << 1) - (t & (stride - 1))
+            _comparator(key, val, pos, pos + stride, ddd)
+            stride >>= 1
+        size <<= 1
+    _elim_and_merge(key, val, half, k, qk)
+    tt = np.arange(k)
+    if block & 1 == 0:
+        return key[tt].copy(), val[tt].copy()
+    return key[k - tt - 1].copy(), val[k - tt - 1].copy()
+
+
+def _bitonic_shared2(src_key, src_val, objects, k, qk, block, size_limit):
+    # src/cuknnBitonic.cu:123-189 for one (block, query)
+    half = size_limit // 2
+    t = np.arange(half)
+    key = np.empty(size_limit, dtype=np.float32)
+    val = np.empty(size_limit, dtype=np.float32)
+    for off in (0, half):
+        g = block * size_limit + t + off
+        ok = g < objects
+        gi = np.minimum(g, len(src_key) - 1)
+        key[t + off] = np.where(ok, src_key[gi], FLT_MAX)
+        val[t + off] = np.where(ok, src_val[gi], FLT_MAX)
+    _elim_and_merge(key, val, half, k, qk)
+    tt = np.arange(k)
+    if block & 1 == 0:
+        return key[tt].copy(), val[tt].copy()
+    return key[k - tt - 1].copy(), val[k - tt - 1].copy()
+
+
+def ref_tbis_select(keys: np.ndarray, k: int, id_offset: int = 0, limit: int = 1024):
+    """Reference truncated bitonic selection (BitonicSelect, src/cuknnBitonic.cu:207-254) on one
+    key row. k must be a power of two <= limit/2 (readme.txt:80). Returns (keys, float idx)."""
+    assert k >= 1 and (k & (k - 1)) == 0 and k <= limit // 2
+    qk = int(np.log2(k))
+    row = np.asarray(keys, dtype=np.float32)
+    m = row.shape[0]
+    grd = (m + limit - 1) // limit
+    parts = [_bitonic_shared(row, k, qk, b, limit, id_offset) for b in range(grd)]
+    cur_k = np.concatenate([p[0] for p in parts])
+    cur_v = np.concatenate([p[1] for p in parts])
+    robjects = grd * k
+    objects = ((robjects + limit - 1) // limit) * limit
+    while robjects > k:
+        bs = min(limit, objects)
+        nb = objects // bs
+        parts = [_bitonic_shared2(cur_k, cur_v, robjects, k, qk, b, bs) for b in range(nb)]
+        cur_k = np.concatenate([p[0] for p in parts])
+        cur_v = np.concatenate([p[1] for p in parts])
+        objects = k * nb
+        robjects = objects
+        objects = ((objects + limit - 1) // limit) * limit
+    return cur_k[:k].copy(), cur_v[:k].copy()           # relloc (:192-204)
+
+
+def ref_merge_res(data: np.ndarray, idx: np.ndarray, k: int, Q: int, P: int):
+    """mergeResBitonic (src/cuknnBitonic.cu:381-417): merge P partial k-lists per query in place.
+    data/idx are flat arrays laid out [P][Q][k]; the result (unsorted) is in partition 0."""
+    data = data.astype(np.float32).copy()
+    idx = idx.copy()
+    for s in range(Q):
+        cmax = -FLT_MAX
+        maxid = 0
+        for i in range(k):
+            if data[s * k + i] > cmax:
+                cmax = data[s * k + i]
+                maxid = i
+        for p in range(1, P):
+            for j in range(k):
+                tmp = data[s * k + p * Q * k + j]
+                if tmp < cmax:
+                    data[s * k + maxid] = tmp
+                    idx[s * k + maxid] = idx[s * k + p * Q * k + j]
+                    cmax = -FLT_MAX
+                    for q in range(k):
+                        if data[s * k + q] > cmax:
+                            cmax = data[s * k + q]
+                            maxid = q
+    return data[: Q * k].reshape(Q, k), idx[: Q * k].reshape(Q, k)
+
+
+# ----------------------------------------------------------------------------------------------
+# Metrics
+# ----------------------------------------------------------------------------------------------
+def recall_at_k(I_got: np.ndarray, I_true: np.ndarray) -> float:
+    n, k = I_true.shape
+    hit = 0
+    for i in range(n):
+        hit += len(np.intersect1d(I_got[i, :k], I_true[i]))
+    return hit / float(n * k)
+
+
+def max_rel_err(D_got: np.ndarray, D_true: np.ndarray) -> float:
+    D_got = np.asarray(D_got, dtype=np.float64)
+    den = np.maximum(np.abs(D_true), np.finfo(np.float64).tiny)
+    return float(np.max(np.abs(D_got - D_true) / den)) if D_true.size else 0.0

@@ -1,0 +1,37 @@
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+for p in (ROOT / "oracle", ROOT / "knn-toolbox_amd", ROOT):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "slow: longer CPU test")
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    import oracle
+
+    oracle.build()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def knns_mod():
+    import knns
+
+    return knns
+
+
+@pytest.fixture(scope="session")
+def gpu(knns_mod):
+    n = knns_mod.device_count()
+    if n <= 0:
+        pytest.fail("no GPU visible to the HIP runtime (GPU tests must run on the GPU box)")
+    return n

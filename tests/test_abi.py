@@ -1,0 +1,117 @@
+"""CPU checks of the C-ABI boundary: the built libraries load and export every function that
+include/*.h declares, struct layouts match, and host-side logic (planner, argument checks) runs
+without a GPU. No compute call is made here."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+INC = ROOT / "include"
+LIB = ROOT / "knn-toolbox_amd" / "lib"
+
+
+def declared_functions(header: Path) -> list[str]:
+    text = header.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"typedef[^;]*;", "", text, flags=re.S)
+    text = re.sub(r"#[^\n]*", "", text)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{}]*\)\s*;", text)
+    return sorted(set(names))
+
+
+def _lib(name):
+    p = LIB / name
+    if not p.exists():
+        pytest.fail(f"{p} missing: run python knn-toolbox_amd/build.py (or __graft_entry__.build())")
+    return ctypes.CDLL(str(p))
+
+
+def test_header_parser_sees_the_api():
+    f = declared_functions(INC / "cuknns.h")
+    assert {"gpuknnsBitonic", "gpuknnsHeap", "knnsplan", "knnsexecute"} <= set(f)
+    g = declared_functions(INC / "knns_native.h")
+    assert {"knns_corpus_create", "knns_search", "knns_last_error"} <= set(g)
+
+
+@pytest.mark.parametrize("lib", ["libcuknn.so", "libcuknn_double.so"])
+def test_dropin_exports(lib):
+    L = _lib(lib)
+    for fn in declared_functions(INC / "cuknns.h"):
+        assert hasattr(L, fn), f"{lib} does not export {fn}"
+
+
+def test_native_exports():
+    L = _lib("libknns_amd.so")
+    for fn in declared_functions(INC / "knns_native.h"):
+        assert hasattr(L, fn), f"libknns_amd.so does not export {fn}"
+
+
+def test_dropin_symbols_are_c_linkage():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(LIB / "libcuknn.so")],
+                         capture_output=True, text=True, check=True).stdout
+    syms = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    for fn in ("gpuknnsBitonic", "gpuknnsHeap", "knnsplan", "knnsexecute"):
+        assert fn in syms
+
+
+def test_plan_struct_layout_matches_header():
+    # knnplan = {fn ptr, 5 ints} (cuknns.h:34-41): 8 + 5*4 -> 32 bytes with tail padding
+    class KnnPlan(ctypes.Structure):
+        _fields_ = [("pt2Function", ctypes.c_void_p), ("objects", ctypes.c_int), ("numQueries", ctypes.c_int),
+                    ("dimentions", ctypes.c_int), ("k", ctypes.c_int), ("numStreams", ctypes.c_int)]
+    L = _lib("libcuknn.so")
+    L.knnsplan.argtypes = [ctypes.POINTER(KnnPlan), ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long]
+    L.knnsplan.restype = None
+    p = KnnPlan()
+    L.knnsplan(ctypes.byref(p), 16384, 1024, 128, 16)  # table-driven: no GPU work
+    assert (p.objects, p.numQueries, p.dimentions, p.k, p.numStreams) == (16384, 1024, 128, 16, 1)
+    heap = ctypes.cast(L.gpuknnsHeap, ctypes.c_void_p).value
+    bit = ctypes.cast(L.gpuknnsBitonic, ctypes.c_void_p).value
+    assert p.pt2Function in (heap, bit)
+
+
+def test_native_params_layout(knns_mod):
+    p = knns_mod.Params()
+    knns_mod.native().knns_params_init(ctypes.byref(p), 1, 7)
+    assert (p.precision, p.k, p.select, p.shortlist, p.flags) == (1, 7, 0, 0, 0)
+    assert ctypes.sizeof(knns_mod.Params) == 48
+
+
+def test_planner_shapes(knns_mod):
+    s = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")
+    assert s["qb"] == 2 and s["dc"] == 128 and s["dp"] == 128
+    assert s["groups"] == (10_000 + 63) // 64
+    assert s["rounds"] >= 3 and 1 <= s["slices_max"] <= 128
+    s = knns_mod.plan_describe(16384, 43, 1024, 16, "fp32")
+    assert s["qb"] == 1 and s["dc"] == 64 and s["dp"] == 64 and s["groups"] == 32
+    s = knns_mod.plan_describe(1000, 2048, 10, 8, "fp32")
+    assert s["dc"] == 128 and s["dp"] == 2048
+
+
+def test_native_argument_errors_without_gpu(knns_mod):
+    L = knns_mod.native()
+    h = ctypes.c_void_p()
+    import numpy as np
+    x = np.zeros((4, 4), np.float32)
+    # invalid shapes are rejected before any device work
+    assert L.knns_corpus_create(ctypes.byref(h), x.ctypes.data, 0, 0, 4, 0) == knns_mod.KNNS_ERR_INVALID
+    assert b"out of range" in L.knns_last_error()
+    assert L.knns_corpus_create(ctypes.byref(h), x.ctypes.data, 0, 4, 0, 0) == knns_mod.KNNS_ERR_INVALID
+    assert L.knns_corpus_create(ctypes.byref(h), x.ctypes.data, 7, 4, 4, 0) == knns_mod.KNNS_ERR_INVALID
+    assert L.knns_corpus_create(ctypes.byref(h), None, 0, 4, 4, 0) == knns_mod.KNNS_ERR_INVALID
+    s = knns_mod.Stats()
+    assert L.knns_plan_describe(10, 4, 1, 2000, 0, ctypes.byref(s)) == knns_mod.KNNS_ERR_INVALID
+
+
+def test_dropin_rejects_bad_sizes_without_gpu(knns_mod):
+    import numpy as np
+    L = knns_mod.dropin(False)
+    q = np.zeros((2, 4), np.float32)
+    out = np.zeros((2, 4), np.float32)
+    t = L.gpuknnsHeap(q.ctypes.data, q.ctypes.data, out.ctypes.data, out.ctypes.data, 0, 2, 4, 4, 1)
+    assert t >= float(np.finfo(np.float32).max)
+    assert b"invalid sizes" in L.knnsGetLastError()
