@@ -162,7 +162,7 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : PREC_F32;
   P.qb = (P.prec_kernel == PREC_BF16 && n > 32) ? 2 : 1;
   const int qw = 32 * P.qb;
-  P.groups = (int)((n + qw - 1) / qw);
+  P.groups = (int)round_up((n + qw - 1) / qw, 4);  // a scan workgroup runs 4 query groups
   P.n_pad = P.groups * qw;
   P.kmax = kmax;
   P.cmax = cmax;

@@ -15,6 +15,8 @@
 // MFMA precision and ||r~||^2 is the exact norm of the rounded row. The reference ranks
 // ||r||^2 - 2 q.r (include/utils.h:957-985), i.e. 2s. Using the norm of the rounded row makes s
 // exactly (||q~ - r~||^2 - ||q~||^2)/2, so the shortlist error scales with ||q - r||, not ||q||.
+#include <type_traits>
+
 #include "knns_internal.h"
 
 namespace knns {
@@ -68,15 +70,28 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane) {
 __device__ __forceinline__ int bits64(uint64_t x) { return x ? 64 - __clzll(x) : 0; }
 
 // ------------------------------------------------------------------------------------------------
-// block-level K-th smallest of N distinct 64-bit entries (radix select, 8 bits per level).
-// Works for any blockDim multiple of 64. get(i) returns entry i. Requires 1 <= K <= N.
+// K-th smallest of N distinct 64-bit entries (radix select, 8 bits per level), by one wave
+// (WAVE) or by the whole block (blockDim multiple of 64, <= 256). get(i) returns entry i. Requires 1 <= K <= N.
 // lds: >= 256 + 16 uint32 of shared scratch.
 // ------------------------------------------------------------------------------------------------
-template <class Get>
-__device__ uint64_t block_kth(Get get, int N, int K, uint32_t* lds) {
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool WAVE>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (WAVE) wave_sync(); else __syncthreads();
+}
+
+template <bool WAVE, class Get>
+__device__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
   uint32_t* hist = lds;                          // 256 bins
   uint64_t* red = (uint64_t*)(lds + 256);        // 2 x (waves) u64, waves <= 4
-  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int nt = WAVE ? 64 : (int)blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   uint64_t mn = ~0ull, mx = 0;
   for (int i = tid; i < N; i += nt) {
     uint64_t e = get(i);
@@ -86,13 +101,13 @@ __device__ uint64_t block_kth(Get get, int N, int K, uint32_t* lds) {
   mn = wave_min_u64(mn);
   mx = wave_max_u64(mx);
   if (lane == 0) { red[wv] = mn; red[4 + wv] = mx; }
-  __syncthreads();
+  group_sync<WAVE>();
   mn = red[0]; mx = red[4];
   for (int w = 1; w < nw; ++w) {
     mn = red[w] < mn ? red[w] : mn;
     mx = red[4 + w] > mx ? red[4 + w] : mx;
   }
-  __syncthreads();
+  group_sync<WAVE>();
   if (K >= N) return mx;
   if (K <= 1) return mn;
   uint64_t base = mn;
@@ -102,7 +117,7 @@ __device__ uint64_t block_kth(Get get, int N, int K, uint32_t* lds) {
   uint64_t result = mx;
   for (int level = 0; level < 9; ++level) {
     for (int b = tid; b < 256; b += nt) hist[b] = 0;
-    __syncthreads();
+    group_sync<WAVE>();
     for (int i = tid; i < N; i += nt) {
       uint64_t e = get(i);
       if (e >= base) {
@@ -110,7 +125,7 @@ __device__ uint64_t block_kth(Get get, int N, int K, uint32_t* lds) {
         if (r < 256) atomicAdd(&hist[(int)r], 1u);
       }
     }
-    __syncthreads();
+    group_sync<WAVE>();
     // wave 0 finds the bin holding the K-th entry
     if (wv == 0) {
       uint32_t c0 = hist[lane * 4 + 0], c1 = hist[lane * 4 + 1], c2 = hist[lane * 4 + 2],
@@ -135,11 +150,11 @@ __device__ uint64_t block_kth(Get get, int N, int K, uint32_t* lds) {
         red[2] = (uint64_t)cb;       // entries in the chosen bin
       }
     }
-    __syncthreads();
+    group_sync<WAVE>();
     const int bstar = (int)red[0];
     const int blw = (int)red[1];
     const int cbin = (int)red[2];
-    __syncthreads();
+    group_sync<WAVE>();
     below += blw;
     base += (uint64_t)bstar << sh;
     if (sh == 0) { result = base; break; }
@@ -153,10 +168,10 @@ __device__ uint64_t block_kth(Get get, int N, int K, uint32_t* lds) {
       }
       m2 = wave_max_u64(m2);
       if (lane == 0) red[4 + wv] = m2;
-      __syncthreads();
+      group_sync<WAVE>();
       m2 = red[4];
       for (int w = 1; w < nw; ++w) m2 = red[4 + w] > m2 ? red[4 + w] : m2;
-      __syncthreads();
+      group_sync<WAVE>();
       result = m2;
       break;
     }
@@ -266,7 +281,7 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
       if (lane == 0) atomicAdd(&a.flags[1], 1);
       if (N <= K) continue;
       auto get = [&](int i) -> uint64_t { return i < n0 ? p0[i] : p1[i - n0]; };
-      const uint64_t E = block_kth(get, N, K, lds);
+      const uint64_t E = group_kth<true>(get, N, K, lds);
       // in-place order-preserving compaction of the logical array [p0 | p1] (writes never
       // overtake reads: a chunk is read completely before any of its outputs is stored)
       int outpos = 0;
@@ -298,22 +313,24 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
 template <int QB, bool FULL>
 __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& st,
                                               const f32x16 (&acc)[QB][2], long t0, long c1) {
-  const int lane = threadIdx.x;
-  const int h = lane >> 5;
+  const int h4 = ((threadIdx.x & 63) >> 5) * 4;
+  const int lim = FULL ? 64 : (int)(c1 - t0);   // valid rows of this tile
+  const uint32_t tb = (uint32_t)t0;
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
+    const int self_rel = st.selfid[qb] < 0 ? -1 : st.selfid[qb] - (int)t0;
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const float D = acc[qb][cb][v];
-        const long row = t0 + cb * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+        const int rel = cb * 32 + 8 * (v >> 2) + (v & 3) + h4;  // row within the tile
         bool ok = D > st.phi[qb];
-        if (!FULL) ok = ok && row < c1;
+        if (!FULL) ok = ok && rel < lim;
         if (ok) {
-          if ((int)row != st.selfid[qb]) {
-            const float s = -D + 0.0f;
-            const uint64_t e = ((uint64_t)f2ord(s) << 32) | (uint32_t)row;
+          if (rel != self_rel) {
+            const float sk = -D + 0.0f;
+            const uint64_t e = ((uint64_t)f2ord(sk) << 32) | (uint32_t)(tb + (uint32_t)rel);
             if (st.cnt[qb] < a.sub_cap) {
               a.sub[st.pbase[qb] + st.cnt[qb]] = e;
               st.cnt[qb] += 1;
@@ -327,12 +344,39 @@ __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& 
   }
 }
 
+// LDS tile of 64 corpus rows x one chunk of DC dims (+ the augmentation columns). The row stride
+// is padded to an odd multiple of 4 dwords so the 16-byte MFMA operand reads of 8 (32 banks) or
+// 16 (64 banks) consecutive rows hit distinct banks.
+template <int PREC, int DC>
+struct TileGeom {
+  static constexpr int ESZ = PREC == PREC_F32 ? 4 : 2;
+  static constexpr int AUG = PREC == PREC_F32 ? 4 : 16;
+  static constexpr int VEC = 16 / ESZ;
+  static constexpr int ROW_EL = DC + AUG;
+  static constexpr int ROW_DW0 = ROW_EL * ESZ / 4;
+  static constexpr int STRIDE_DW = ROW_DW0 + ((4 - ROW_DW0 % 8) + 8) % 8;
+  static constexpr int STRIDE_EL = STRIDE_DW * 4 / ESZ;
+  static constexpr int UNITS = ROW_EL / VEC;
+  static constexpr int MAIN_UNITS = DC / VEC;
+  static constexpr int TILE_UNITS = 64 * UNITS;
+  static constexpr int LOADS = (TILE_UNITS + 255) / 256;
+  static constexpr int BUF_DW = 64 * STRIDE_DW;
+  static_assert(STRIDE_DW % 8 == 4, "stride must be an odd multiple of 4 dwords");
+};
+
+// One workgroup = 4 waves = 4 query groups (32*QB queries each) sharing one corpus slice. The
+// slice is streamed as (tile, chunk) stages through two LDS buffers: while the waves run the MFMAs
+// of stage s, the global loads of stage s+1 are in flight into registers.
 template <int PREC, int QB, int DC>
-__global__ __launch_bounds__(64) void scan_kernel(ScanArgs a) {
-  __shared__ __attribute__((aligned(16))) uint32_t lds[256 + 32];
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
+  using G = TileGeom<PREC, DC>;
+  __shared__ __attribute__((aligned(16))) uint32_t tile[2][G::BUF_DW];
+  __shared__ __attribute__((aligned(16))) uint32_t kth_lds[4][256 + 32];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
   const int col = lane & 31, h = lane >> 5;
-  const int g = blockIdx.x, sl = blockIdx.y;
+  const int g = blockIdx.x * 4 + wv;   // query group of this wave
+  const int sl = blockIdx.y;
   const long c0 = a.row_begin + (long)sl * a.slice_rows;
   long c1 = c0 + a.slice_rows;
   if (c1 > a.row_end) c1 = a.row_end;
@@ -347,125 +391,143 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs a) {
     st.pbase[qb] = (((long)q * a.slices_max + sl) * 2 + h) * a.sub_cap;
     st.selfid[qb] = a.self_ids ? a.self_ids[q] : -1;
   }
-
   const int nch = a.dp / DC;
-  if constexpr (PREC == PREC_F32) {
-    const float* X = (const float*)a.X;
-    const float* Q = (const float*)a.Qp;
-    constexpr int NB = DC / 2;          // B floats per lane per query block per chunk
-    float bq[QB][NB];
-    auto loadB = [&](int c) {
+  const long ntiles = c0 < c1 ? (c1 - c0 + 63) / 64 : 0;
+  const long nstage = ntiles * nch;
+
+  typedef typename std::conditional<PREC == PREC_F32, float, uint16_t>::type T;
+  const T* X = (const T*)a.X;
+  const T* Q = (const T*)a.Qp;
+
+  // ---- stage loader: global -> LDS by DMA (global_load_lds_dwordx4) ----
+  // The buffer is a dense array of 16-byte units, SU per row (the row padding is one unit). Load
+  // instruction l of wave wv fills units [(wv + 4l)*64, +64): lane -> (row, unit) -> global source.
+  // Padding units re-read the row's first unit.
+  constexpr int SU = G::STRIDE_DW / 4;
+  constexpr int WLOADS = (SU + 3) / 4;
+  auto load_stage = [&](long s, int buf) {
+    const long t0 = c0 + (s / nch) * 64;
+    const int c = (int)(s % nch);
 #pragma unroll
-      for (int qb = 0; qb < QB; ++qb) {
-        const float* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + c * DC + h * (DC / 2);
+    for (int l = 0; l < WLOADS; ++l) {
+      const int blk = wv + 4 * l;
+      if (blk < SU) {
+        const int U = blk * 64 + lane;
+        const int r = U / SU, u = U % SU;
+        const T* row = X + (t0 + r) * a.ldx;
+        const T* src = u < G::MAIN_UNITS ? row + (long)c * DC + u * G::VEC
+                     : (u < G::UNITS ? row + a.dp + (u - G::MAIN_UNITS) * G::VEC : row);
+        __builtin_amdgcn_global_load_lds((const void*)src,
+            (void __attribute__((address_space(3)))*)&tile[buf][blk * 256], 16, 0, 0);
+      }
+    }
+  };
+  static_assert(G::STRIDE_DW % 4 == 0, "rows are whole 16-byte units");
+
+  // ---- per-wave query operand (B) ----
+  constexpr int NB = PREC == PREC_F32 ? DC / 2 : DC / 16;
+  typedef typename std::conditional<PREC == PREC_F32, float, bf16x8>::type BT;
+  BT bq[QB][NB];
+  auto loadB = [&](int c) {
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      const T* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + c * DC + h * (DC / 2);
+      if constexpr (PREC == PREC_F32) {
 #pragma unroll
         for (int j = 0; j < NB; j += 4) {
           const f32x4 t = *(const f32x4*)(qp + j);
           bq[qb][j] = t[0]; bq[qb][j + 1] = t[1]; bq[qb][j + 2] = t[2]; bq[qb][j + 3] = t[3];
         }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) bq[qb][j] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 8 * j));
       }
-    };
-    if (nch == 1) loadB(0);
-    for (long t0 = c0; t0 < c1; t0 += 64) {
-      f32x16 acc[QB][2];
+    }
+  };
+  BT baug[QB];
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const T* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + a.dp;
+    if constexpr (PREC == PREC_F32) baug[qb] = qp[h];
+    else baug[qb] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 8 * h));
+  }
+  if (nch == 1) loadB(0);
+
+  if (nstage > 0) load_stage(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  f32x16 acc[QB][2];
+  for (long s = 0; s < nstage; ++s) {
+    const int buf = (int)(s & 1);
+    const int c = (int)(s % nch);
+    const long t0 = c0 + (s / nch) * 64;
+    if (s + 1 < nstage) load_stage(s + 1, buf ^ 1);
+    if (nch > 1) loadB(c);
+    if (c == 0) {
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb)
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) acc[qb][cb] = (f32x16){0};
-      const float* xr0 = X + (t0 + col) * a.ldx;
-      const float* xr1 = xr0 + 32 * a.ldx;
-      for (int c = 0; c < nch; ++c) {
-        if (nch > 1) loadB(c);
-        const float* x0 = xr0 + c * DC + h * (DC / 2);
-        const float* x1 = xr1 + c * DC + h * (DC / 2);
-#pragma unroll
-        for (int j = 0; j < NB; j += 4) {
-          const f32x4 a0 = *(const f32x4*)(x0 + j);
-          const f32x4 a1 = *(const f32x4*)(x1 + j);
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-#pragma unroll
-            for (int qb = 0; qb < QB; ++qb) {
-              acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[jj], bq[qb][j + jj], acc[qb][0], 0, 0, 0);
-              acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[jj], bq[qb][j + jj], acc[qb][1], 0, 0, 0);
-            }
-          }
-        }
-      }
-      {  // augmentation: -||r||^2/2 as hi + lo
-        const float g0 = xr0[a.dp + h], g1 = xr1[a.dp + h];
-#pragma unroll
-        for (int qb = 0; qb < QB; ++qb) {
-          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(g0, 1.0f, acc[qb][0], 0, 0, 0);
-          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(g1, 1.0f, acc[qb][1], 0, 0, 0);
-        }
-      }
-      if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
-      else scan_epilogue<QB, false>(a, st, acc, t0, c1);
-      bool trig = false;
-#pragma unroll
-      for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 32;
-      if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, lds);
     }
-  } else {  // BF16
-    const uint16_t* X = (const uint16_t*)a.X;
-    const uint16_t* Q = (const uint16_t*)a.Qp;
-    constexpr int NS = DC / 16;         // MFMA steps per chunk
-    bf16x8 bq[QB][NS];
-    auto loadB = [&](int c) {
+    const uint32_t* r0 = &tile[buf][col * G::STRIDE_DW];
+    const uint32_t* r1 = &tile[buf][(32 + col) * G::STRIDE_DW];
+    const int hoff = h * (DC / 2) * G::ESZ / 4;  // dword offset of this lane half
+    if constexpr (PREC == PREC_F32) {
 #pragma unroll
-      for (int qb = 0; qb < QB; ++qb) {
-        const uint16_t* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + c * DC + h * (DC / 2);
+      for (int j = 0; j < DC / 2; j += 4) {
+        const f32x4 a0 = __builtin_bit_cast(f32x4, *(const u32x4*)(r0 + hoff + j));
+        const f32x4 a1 = __builtin_bit_cast(f32x4, *(const u32x4*)(r1 + hoff + j));
 #pragma unroll
-        for (int j = 0; j < NS; ++j) bq[qb][j] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 8 * j));
-      }
-    };
-    if (nch == 1) loadB(0);
-    bf16x8 baug[QB];
-#pragma unroll
-    for (int qb = 0; qb < QB; ++qb) {
-      const uint16_t* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + a.dp + 8 * h;
-      baug[qb] = __builtin_bit_cast(bf16x8, *(const u32x4*)qp);
-    }
-    for (long t0 = c0; t0 < c1; t0 += 64) {
-      f32x16 acc[QB][2];
-#pragma unroll
-      for (int qb = 0; qb < QB; ++qb)
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) acc[qb][cb] = (f32x16){0};
-      const uint16_t* xr0 = X + (t0 + col) * a.ldx;
-      const uint16_t* xr1 = xr0 + 32 * a.ldx;
-      for (int c = 0; c < nch; ++c) {
-        if (nch > 1) loadB(c);
-        const uint16_t* x0 = xr0 + c * DC + h * (DC / 2);
-        const uint16_t* x1 = xr1 + c * DC + h * (DC / 2);
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-          const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(x0 + 8 * j));
-          const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(x1 + 8 * j));
+        for (int jj = 0; jj < 4; ++jj) {
 #pragma unroll
           for (int qb = 0; qb < QB; ++qb) {
-            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[qb][j], acc[qb][0], 0, 0, 0);
-            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[qb][j], acc[qb][1], 0, 0, 0);
+            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[jj], bq[qb][j + jj], acc[qb][0], 0, 0, 0);
+            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[jj], bq[qb][j + jj], acc[qb][1], 0, 0, 0);
           }
         }
       }
-      {
-        const bf16x8 g0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(xr0 + a.dp + 8 * h));
-        const bf16x8 g1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(xr1 + a.dp + 8 * h));
+      if (c == nch - 1) {
+        const float g0 = __builtin_bit_cast(float, r0[DC + h]);
+        const float g1 = __builtin_bit_cast(float, r1[DC + h]);
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(g0, baug[qb], acc[qb][0], 0, 0, 0);
+          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(g1, baug[qb], acc[qb][1], 0, 0, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < DC / 16; ++j) {
+        const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r0 + hoff + 4 * j));
+        const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r1 + hoff + 4 * j));
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[qb][j], acc[qb][0], 0, 0, 0);
+          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[qb][j], acc[qb][1], 0, 0, 0);
+        }
+        // keep at most two steps of operands in flight (register budget of 2 waves/SIMD)
+        if ((j & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+      }
+      if (c == nch - 1) {
+        const bf16x8 g0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r0 + DC / 2 + 4 * h));
+        const bf16x8 g1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r1 + DC / 2 + 4 * h));
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) {
           acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g0, baug[qb], acc[qb][0], 0, 0, 0);
           acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g1, baug[qb], acc[qb][1], 0, 0, 0);
         }
       }
+    }
+    if (c == nch - 1) {
       if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
       else scan_epilogue<QB, false>(a, st, acc, t0, c1);
       bool trig = false;
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 32;
-      if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, lds);
+      if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv]);
     }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into the other buffer has landed
+    __syncthreads();
   }
   // publish fills
 #pragma unroll
@@ -555,7 +617,7 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
     }
     __syncthreads();
     auto lget = [&](int i) -> uint64_t { return ent[i]; };
-    E = block_kth(lget, N, K, scratch);
+    E = group_kth<false>(lget, N, K, scratch);
     for (int i = tid; i < N; i += nt) {
       const uint64_t e = ent[i];
       if (e <= E) {
@@ -564,7 +626,7 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
       }
     }
   } else {
-    E = block_kth(gget, N, K, scratch);
+    E = group_kth<false>(gget, N, K, scratch);
     for (int i = tid; i < N; i += nt) {
       const uint64_t e = gget(i);
       if (e <= E) {
@@ -705,7 +767,8 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
 
 template <int PREC, int QB, int DC>
 static hipError_t scan_one(const ScanArgs& a, int groups, hipStream_t stream) {
-  hipLaunchKernelGGL((scan_kernel<PREC, QB, DC>), dim3(groups, a.slices), dim3(64), 0, stream, a);
+  if (groups % 4) return hipErrorInvalidValue;  // a workgroup runs 4 query groups
+  hipLaunchKernelGGL((scan_kernel<PREC, QB, DC>), dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
