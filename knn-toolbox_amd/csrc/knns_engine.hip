@@ -166,8 +166,11 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   P.n_pad = P.groups * qw;
   P.kmax = kmax;
   P.cmax = cmax;
-  const long target_waves = (long)cus * 4 * 2;
-  int smax = (int)std::min<long>(128, std::max<long>(1, (target_waves + P.groups - 1) / P.groups));
+  // scan workgroups: 4 query groups each, 2 resident per CU; aim for ~4 full waves of workgroups
+  // so the last, partial one is a small tail
+  const long wg_rows = P.groups / 4;
+  const long target_wgs = (long)cus * 2 * 4;
+  int smax = (int)std::min<long>(128, std::max<long>(1, (target_wgs + wg_rows - 1) / wg_rows));
   if (slices_req > 0) smax = std::min(128, slices_req);
   const int ratio = ratio_req >= 2 ? ratio_req : 2;
   long b = 0;

@@ -80,6 +80,13 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// s_waitcnt vmcnt(N) leaving expcnt/lgkmcnt unconstrained (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
 template <bool WAVE>
 __device__ __forceinline__ void group_sync() {
   if constexpr (WAVE) wave_sync(); else __syncthreads();
@@ -310,6 +317,11 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
   }
 }
 
+// Filter of one 64-row tile. Survivors are rare in the large rounds (~0.1% of the pairs), so the
+// common case must cost a few VALU ops per value and no exec-mask juggling: each lane reduces its
+// values in groups of 4 consecutive rows (v_max3 + v_max), one wave-uniform ballot per query
+// block skips the block, one per group skips the group, and only groups holding a survivor run
+// the per-value divergent append.
 template <int QB, bool FULL>
 __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& st,
                                               const f32x16 (&acc)[QB][2], long t0, long c1) {
@@ -318,16 +330,34 @@ __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& 
   const uint32_t tb = (uint32_t)t0;
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
-    const int self_rel = st.selfid[qb] < 0 ? -1 : st.selfid[qb] - (int)t0;
+    float Dv[2][16];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
+    for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const float D = acc[qb][cb][v];
-        const int rel = cb * 32 + 8 * (v >> 2) + (v & 3) + h4;  // row within the tile
-        bool ok = D > st.phi[qb];
-        if (!FULL) ok = ok && rel < lim;
-        if (ok) {
+        const int rel = cb * 32 + 8 * (v >> 2) + (v & 3) + h4;
+        Dv[cb][v] = (FULL || rel < lim) ? acc[qb][cb][v] : -__builtin_inff();
+      }
+    float gm[8];
+#pragma unroll
+    for (int gi = 0; gi < 8; ++gi) {
+      const int cb = gi >> 2, v0 = (gi & 3) * 4;
+      gm[gi] = fmaxf(fmaxf(Dv[cb][v0], Dv[cb][v0 + 1]), fmaxf(Dv[cb][v0 + 2], Dv[cb][v0 + 3]));
+    }
+    const float qm = fmaxf(fmaxf(fmaxf(gm[0], gm[1]), fmaxf(gm[2], gm[3])),
+                           fmaxf(fmaxf(gm[4], gm[5]), fmaxf(gm[6], gm[7])));
+    const float phi = st.phi[qb];
+    if (__builtin_amdgcn_ballot_w64(qm > phi) == 0) continue;
+    const int self_rel = st.selfid[qb] < 0 ? -1 : st.selfid[qb] - (int)t0;
+#pragma unroll
+    for (int gi = 0; gi < 8; ++gi) {
+      if (__builtin_amdgcn_ballot_w64(gm[gi] > phi) == 0) continue;
+      const int cb = gi >> 2, v0 = (gi & 3) * 4;
+#pragma unroll
+      for (int v = v0; v < v0 + 4; ++v) {
+        const float D = Dv[cb][v];
+        if (D > phi) {
+          const int rel = cb * 32 + 8 * (v >> 2) + (v & 3) + h4;
           if (rel != self_rel) {
             const float sk = -D + 0.0f;
             const uint64_t e = ((uint64_t)f2ord(sk) << 32) | (uint32_t)(tb + (uint32_t)rel);
@@ -370,7 +400,11 @@ struct TileGeom {
 template <int PREC, int QB, int DC>
 __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   using G = TileGeom<PREC, DC>;
-  __shared__ __attribute__((aligned(16))) uint32_t tile[2][G::BUF_DW];
+  // BF16 tiles compute in ~1.2k cycles at peak, shorter than a DMA round trip: keep two stages in
+  // flight. FP32 tiles take 8x longer; one stage of lookahead hides the latency.
+  constexpr int NBUF = PREC == PREC_F32 ? 2 : 3;
+  constexpr int PF = NBUF - 1;   // prefetch distance in stages
+  __shared__ __attribute__((aligned(16))) uint32_t tile[NBUF][G::BUF_DW];
   __shared__ __attribute__((aligned(16))) uint32_t kth_lds[4][256 + 32];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
@@ -410,8 +444,10 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
     const int c = (int)(s % nch);
 #pragma unroll
     for (int l = 0; l < WLOADS; ++l) {
-      const int blk = wv + 4 * l;
-      if (blk < SU) {
+      // blocks past the end repeat the last block (same source, same destination, same bytes) so
+      // every wave has exactly WLOADS loads in flight per stage and vmcnt waits can count stages
+      const int blk = wv + 4 * l < SU ? wv + 4 * l : SU - 1;
+      {
         const int U = blk * 64 + lane;
         const int r = U / SU, u = U % SU;
         const T* row = X + (t0 + r) * a.ldx;
@@ -453,15 +489,21 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   }
   if (nch == 1) loadB(0);
 
-  if (nstage > 0) load_stage(0, 0);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PF; ++p)
+    if (p < nstage) load_stage(p, p);
   f32x16 acc[QB][2];
   for (long s = 0; s < nstage; ++s) {
-    const int buf = (int)(s & 1);
+    // stage s must have landed; the loads of the younger stages may stay in flight (VMEM
+    // operations retire in issue order, later stores only make this wait longer)
+    const long younger = nstage - 1 - s < PF - 1 ? nstage - 1 - s : PF - 1;
+    if (younger >= 1) wait_vmcnt<WLOADS>();
+    else __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();   // every wave's DMA for stage s is visible; buffer of stage s-1 is free
+    const int buf = (int)(s % NBUF);
     const int c = (int)(s % nch);
     const long t0 = c0 + (s / nch) * 64;
-    if (s + 1 < nstage) load_stage(s + 1, buf ^ 1);
+    if (s + PF < nstage) load_stage(s + PF, (int)((s + PF) % NBUF));
     if (nch > 1) loadB(c);
     if (c == 0) {
 #pragma unroll
@@ -526,8 +568,6 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
       for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 32;
       if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv]);
     }
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into the other buffer has landed
-    __syncthreads();
   }
   // publish fills
 #pragma unroll
