@@ -168,7 +168,7 @@ def main() -> int:
         "data": "synthetic (1024-centre Gaussian mixture, numpy PCG64 seeds 2/3)",
         "config": {"workload": "C2: corpus 1,000,000 x 128 FP32 (BF16 copy), 10,000 queries, k=100",
                    "m": m, "n": n, "d": d, "k": k,
-                   "shortlist": args.shortlist or 2 * k + 32, "parallelism": "single GPU"},
+                   "shortlist": args.shortlist or k + k // 2 + 16, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None},
         "cpu_baseline": cpu,

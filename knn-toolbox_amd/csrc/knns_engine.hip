@@ -139,7 +139,7 @@ void dims_for(int d, int* dp, int* dc) {
 
 int default_shortlist(int prec, int k) {
   switch (prec) {
-    case KNNS_PREC_BF16: return 2 * k + 32;
+    case KNNS_PREC_BF16: return k + k / 2 + 16;
     case KNNS_PREC_FP8: return 4 * k + 32;
     default: return k + 16;
   }
@@ -172,7 +172,9 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   const long target_wgs = (long)cus * 2 * 4;
   int smax = (int)std::min<long>(128, std::max<long>(1, (target_wgs + wg_rows - 1) / wg_rows));
   if (slices_req > 0) smax = std::min(128, slices_req);
-  const int ratio = ratio_req >= 2 ? ratio_req : 2;
+  // growth factor of the corpus prefix between threshold rounds: a fixed ratio if requested,
+  // else x4 while the prefix is under m/8 (cheap rounds, fewer merges) and x2 after (the large
+  // rounds, where a tighter threshold saves filter work)
   long b = 0;
   long n0 = std::min<long>(m, round_up(std::max<long>(2L * cmax, 64), 64));
   long e = n0;
@@ -188,6 +190,7 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
     smax_used = std::max(smax_used, r.slices);
     P.rounds.push_back(r);
     b = e;
+    const long ratio = ratio_req >= 2 ? ratio_req : (e * 8 < m ? 4 : 2);
     e = std::min<long>(m, round_up(e * ratio, 64));
   }
   P.slices_max = smax_used;
@@ -370,6 +373,8 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     sa.sub_cap = P.sub_cap;
     sa.self_ids = (const int*)selfd;
     sa.flags = (int*)flags;
+    sa.wg_rows = P.groups / 4;
+    sa.xcd_swizzle = (R.slices >= 8 && std::getenv("KNNS_XCD_SWIZZLE")) ? 1 : 0;
     const size_t e0 = ev++, e1 = ev++;
     KNNS_HIP(hipEventRecord(E.event(e0), st));
     KNNS_HIP(launch_scan(P.prec_kernel, P.qb, P.dc, sa, P.groups, st));

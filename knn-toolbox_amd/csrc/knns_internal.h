@@ -47,6 +47,8 @@ struct ScanArgs {
   int sub_cap;
   const int* self_ids;   // optional: corpus row excluded for each query (all-kNN), -1 = none
   int* flags;            // [0] = overflow counter, [1] = slow-path compactions
+  int xcd_swizzle;       // 1: 1-D grid, slice sl runs on XCD sl % 8 (see scan_kernel)
+  int wg_rows;           // query workgroup rows (groups / 4) when xcd_swizzle
 };
 
 struct MergeArgs {

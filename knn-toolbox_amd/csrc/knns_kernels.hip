@@ -409,8 +409,21 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int col = lane & 31, h = lane >> 5;
-  const int g = blockIdx.x * 4 + wv;   // query group of this wave
-  const int sl = blockIdx.y;
+  // Workgroups are dispatched round-robin over the 8 XCDs (linear id % 8). With a_swizzle the
+  // linear id is mapped so that every workgroup of slice sl runs on XCD sl % 8: the query rows of a
+  // slice then share that XCD's L2 instead of every XCD fetching every slice.
+  int wgrow, sl;
+  if (a.xcd_swizzle) {
+    const int id = blockIdx.x;
+    const int xcd = id & 7, local = id >> 3;
+    wgrow = local % a.wg_rows;
+    sl = xcd + 8 * (local / a.wg_rows);
+    if (sl >= a.slices) return;   // padding workgroup (slices rounded up to a multiple of 8)
+  } else {
+    wgrow = blockIdx.x;
+    sl = blockIdx.y;
+  }
+  const int g = wgrow * 4 + wv;   // query group of this wave
   const long c0 = a.row_begin + (long)sl * a.slice_rows;
   long c1 = c0 + a.slice_rows;
   if (c1 > a.row_end) c1 = a.row_end;
@@ -578,17 +591,18 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// merge kernel: one 256-thread block per query
+// merge kernel: one wave per query (4 queries per 256-thread block, no block barriers)
 // ------------------------------------------------------------------------------------------------
-constexpr int MERGE_LDS_ENTRIES = 4096;   // 32 KB of entries gathered to LDS
 constexpr int MERGE_MAX_SRC = 1 + 2 * 128;
+constexpr int MW_ENT = 1024;   // entries gathered to LDS per wave (8 KB)
 
-__device__ __forceinline__ void bitonic_sort_lds_u64(uint64_t* v, int P) {
-  const int tid = threadIdx.x, nt = blockDim.x;
+// wave-level bitonic sort of P (power of two <= MW_ENT) u64 entries in LDS, ascending
+__device__ __forceinline__ void wave_bitonic_u64(uint64_t* v, int P) {
+  const int lane = threadIdx.x & 63;
   for (int size = 2; size <= P; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      __syncthreads();
-      for (int t = tid; t < P / 2; t += nt) {
+      wave_sync();
+      for (int t = lane; t < P / 2; t += 64) {
         const int pos = 2 * t - (t & (stride - 1));
         const bool up = ((pos & size) == 0);
         const uint64_t x = v[pos], y = v[pos + stride];
@@ -596,39 +610,48 @@ __device__ __forceinline__ void bitonic_sort_lds_u64(uint64_t* v, int P) {
       }
     }
   }
-  __syncthreads();
+  wave_sync();
 }
 
 __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
-  __shared__ __attribute__((aligned(16))) uint64_t ent[MERGE_LDS_ENTRIES];
-  __shared__ int off[MERGE_MAX_SRC + 1];
-  __shared__ __attribute__((aligned(16))) uint32_t scratch[256 + 32];
-  __shared__ int outc;
-  const int q = blockIdx.x;
-  if (q >= a.n) return;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ __attribute__((aligned(16))) uint64_t ent_all[4][MW_ENT];
+  __shared__ int off_all[4][MERGE_MAX_SRC + 1];
+  __shared__ __attribute__((aligned(16))) uint32_t kth_all[4][256 + 32];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = blockIdx.x * 4 + wv;
+  if (q >= a.n) return;   // wave-uniform; only wave-level synchronisation below
+  uint64_t* ent = ent_all[wv];
+  int* off = off_all[wv];
   const int nsrc = 1 + 2 * a.slices;
   const long subq = (long)q * a.slices_max * 2;
-  // source counts -> exclusive offsets (sequential in one thread: nsrc <= 257)
-  if (tid == 0) {
-    int acc = 0;
-    off[0] = 0;
-    acc += a.carry_in_cnt[q];
-    off[1] = acc;
-    for (int s = 0; s < 2 * a.slices; ++s) {
-      acc += a.sub_cnt[subq + s];
-      off[s + 2] = acc;
+  // source fills -> exclusive offsets: lane l owns sources [l*P, l*P + P)
+  {
+    const int P = (nsrc + 63) / 64;
+    int c[5] = {0, 0, 0, 0, 0};
+    int tot = 0;
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int sidx = lane * P + u;
+      if (u < P && sidx < nsrc) c[u] = sidx == 0 ? a.carry_in_cnt[q] : a.sub_cnt[subq + sidx - 1];
+      tot += c[u];
     }
-    outc = 0;
+    const int excl = (int)wave_incl_scan_u32((uint32_t)tot, lane) - tot;
+    int acc = excl;
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int sidx = lane * P + u;
+      if (u < P && sidx < nsrc) off[sidx] = acc;
+      acc += c[u];
+    }
+    if (lane == 63) off[nsrc] = acc;
+    wave_sync();
   }
-  __syncthreads();
   const int N = off[nsrc];
   const int K = a.kc[q];
   const uint64_t* cin = a.carry_in + (long)q * a.carry_cap;
   const uint64_t* sub = a.sub + subq * a.sub_cap;
   auto gget = [&](int i) -> uint64_t {
-    // binary search the source of logical entry i
-    int lo = 0, hi = nsrc - 1;
+    int lo = 0, hi = nsrc - 1;     // last source whose offset is <= i
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (off[mid] <= i) lo = mid; else hi = mid - 1;
@@ -637,50 +660,66 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
     return lo == 0 ? cin[j] : sub[(long)(lo - 1) * a.sub_cap + j];
   };
   uint64_t* cout = a.carry_out + (long)q * a.carry_cap;
+  auto compact = [&](auto get, uint64_t E) {
+    int outpos = 0;
+    for (int base = 0; base < N; base += 64) {
+      const int i = base + lane;
+      uint64_t e = 0;
+      bool keep = false;
+      if (i < N) { e = get(i); keep = e <= E; }
+      const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+      const int pos = outpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+      if (keep && pos < a.carry_cap) cout[pos] = e;
+      outpos += __popcll(km);
+    }
+    return outpos;
+  };
   if (N <= K) {
-    for (int i = tid; i < N; i += nt) cout[i] = gget(i);
-    if (tid == 0) a.carry_out_cnt[q] = N;
+    for (int i = lane; i < N; i += 64) cout[i] = gget(i);
+    if (lane == 0) a.carry_out_cnt[q] = N;
     return;
   }
   uint64_t E;
-  if (N <= MERGE_LDS_ENTRIES) {
-    for (int i = tid; i < N; i += nt) ent[i] = gget(i);
-    if (a.bitonic) {
-      int P = 1;
-      while (P < N) P <<= 1;
-      for (int i = N + tid; i < P; i += nt) ent[i] = ~0ull;
-      bitonic_sort_lds_u64(ent, P);
-      for (int i = tid; i < K; i += nt) cout[i] = ent[i];
-      E = ent[K - 1];
-      if (tid == 0) { a.carry_out_cnt[q] = K; a.thr[q] = (uint32_t)(E >> 32); }
-      return;
-    }
-    __syncthreads();
-    auto lget = [&](int i) -> uint64_t { return ent[i]; };
-    E = group_kth<false>(lget, N, K, scratch);
-    for (int i = tid; i < N; i += nt) {
-      const uint64_t e = ent[i];
-      if (e <= E) {
-        const int pos = atomicAdd(&outc, 1);
-        if (pos < a.carry_cap) cout[pos] = e;
+  int kept;
+  if (N <= MW_ENT) {
+    // gather: independent loads, four per lane in flight
+    for (int base = 0; base < N; base += 256) {
+      uint64_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + u * 64 + lane;
+        v[u] = i < N ? gget(i) : 0;
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + u * 64 + lane;
+        if (i < N) ent[i] = v[u];
+      }
+    }
+    wave_sync();
+    if (a.bitonic) {
+      int P = 64;
+      while (P < N) P <<= 1;
+      for (int i = N + lane; i < P; i += 64) ent[i] = ~0ull;
+      wave_bitonic_u64(ent, P);
+      for (int i = lane; i < K; i += 64) cout[i] = ent[i];
+      E = ent[K - 1];
+      kept = K;
+    } else {
+      auto lget = [&](int i) -> uint64_t { return ent[i]; };
+      E = group_kth<true>(lget, N, K, kth_all[wv]);
+      kept = compact(lget, E);
     }
   } else {
-    E = group_kth<false>(gget, N, K, scratch);
-    for (int i = tid; i < N; i += nt) {
-      const uint64_t e = gget(i);
-      if (e <= E) {
-        const int pos = atomicAdd(&outc, 1);
-        if (pos < a.carry_cap) cout[pos] = e;
-      }
-    }
+    E = group_kth<true>(gget, N, K, kth_all[wv]);
+    kept = compact(gget, E);
   }
-  __syncthreads();
-  if (tid == 0) {
-    if (outc > a.carry_cap) outc = a.carry_cap;
-    a.carry_out_cnt[q] = outc;
+  if (lane == 0) {
+    if (kept > a.carry_cap) kept = a.carry_cap;
+    a.carry_out_cnt[q] = kept;
     a.thr[q] = (uint32_t)(E >> 32);
-    if (outc != K) atomicAdd(&a.flags[0], 1);
+    if (kept != K) atomicAdd(&a.flags[0], 1);
   }
 }
 
@@ -707,7 +746,7 @@ __device__ __forceinline__ void bitonic_sort_lds_pair(uint64_t* k, uint32_t* v, 
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void final_kernel(FinalArgs a) {
+__global__ __launch_bounds__(256) void final_block_kernel(FinalArgs a) {
   __shared__ uint64_t ek[FINAL_MAX];
   __shared__ uint32_t ev[FINAL_MAX];
   __shared__ double qrow[4096];
@@ -719,10 +758,9 @@ __global__ __launch_bounds__(256) void final_kernel(FinalArgs a) {
   int K = a.kc[q];
   const int k = a.kq[q];
   const uint64_t* src = a.carry + (long)q * a.carry_cap;
-  int P = 1;
-  while (P < cnt) P <<= 1;
-  for (int i = tid; i < P; i += nt) ek[i] = i < cnt ? src[i] : ~0ull;
-  bitonic_sort_lds_u64(ek, P);
+  // the merge leaves exactly the shortlist in the carry (at most K entries): rerank all of them
+  for (int i = tid; i < cnt; i += nt) ek[i] = src[i];
+  __syncthreads();
   const int Kq = K < cnt ? K : cnt;
   // query master row
   const int d = a.d;
@@ -783,6 +821,140 @@ __global__ __launch_bounds__(256) void final_kernel(FinalArgs a) {
   }
 }
 
+// Final rerank, one wave per query (shortlists up to FW_MAX). Each lane reranks up to four
+// candidates at a time: four independent FMA chains, each in the canonical sequential order.
+constexpr int FW_MAX = 512;
+
+__device__ __forceinline__ void wave_bitonic_pair(uint64_t* k, uint32_t* v, int P) {
+  const int lane = threadIdx.x & 63;
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      wave_sync();
+      for (int t = lane; t < P / 2; t += 64) {
+        const int pos = 2 * t - (t & (stride - 1));
+        const bool up = ((pos & size) == 0);
+        const uint64_t x = k[pos], y = k[pos + stride];
+        const uint32_t xv = v[pos], yv = v[pos + stride];
+        const bool gt = (x > y) || (x == y && xv > yv);
+        if (gt == up) { k[pos] = y; k[pos + stride] = x; v[pos] = yv; v[pos + stride] = xv; }
+      }
+    }
+  }
+  wave_sync();
+}
+
+__global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
+  __shared__ __attribute__((aligned(16))) uint64_t dk_all[4][FW_MAX];
+  __shared__ uint32_t di_all[4][FW_MAX];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = blockIdx.x * 4 + wv;
+  if (q >= a.n) return;   // wave-uniform
+  uint64_t* dk = dk_all[wv];
+  uint32_t* di = di_all[wv];
+  int cnt = a.carry_cnt[q];
+  if (cnt > FW_MAX) cnt = FW_MAX;
+  const int K = a.kc[q];
+  const int Kq = K < cnt ? K : cnt;
+  const int k = a.kq[q];
+  const int d = a.d;
+  const uint64_t* src = a.carry + (long)q * a.carry_cap;
+  for (int b = 0; b < Kq; b += 256) {
+    uint32_t idx[4];
+    bool ok[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = b + u * 64 + lane;
+      ok[u] = t < Kq;
+      idx[u] = ok[u] ? (uint32_t)(src[t] & 0xffffffffu) : 0u;   // row 0 stands in for idle lanes
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!a.master_f64) {
+      const float* qr = (const float*)a.Qm + (long)q * a.ldqm;
+      const float* r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r[u] = (const float*)a.Xm + (long)idx[u] * a.ldxm;
+      int j = 0;
+      if ((a.ldxm & 3) == 0 && (a.ldqm & 3) == 0) {
+        // 8 x 16 bytes per row per step: 32 independent loads in flight per lane (the gather is
+        // latency-bound, rows are scattered over the corpus)
+        constexpr int U = 8;
+        for (; j + 4 * U <= d; j += 4 * U) {
+          f32x4 qv[U], rv[U][4];
+#pragma unroll
+          for (int w = 0; w < U; ++w) {
+            qv[w] = *(const f32x4*)(qr + j + 4 * w);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rv[w][u] = *(const f32x4*)(r[u] + j + 4 * w);
+          }
+#pragma unroll
+          for (int w = 0; w < U; ++w)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const double t1 = (double)qv[w][e] - (double)rv[w][u][e];
+                acc[u] = __builtin_fma(t1, t1, acc[u]);
+              }
+        }
+        for (; j + 4 <= d; j += 4) {
+          const f32x4 qv = *(const f32x4*)(qr + j);
+          f32x4 rv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) rv[u] = *(const f32x4*)(r[u] + j);
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const double t1 = (double)qv[e] - (double)rv[u][e];
+              acc[u] = __builtin_fma(t1, t1, acc[u]);
+            }
+        }
+      }
+      for (; j < d; ++j) {
+        const double qj = (double)qr[j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double t1 = qj - (double)r[u][j];
+          acc[u] = __builtin_fma(t1, t1, acc[u]);
+        }
+      }
+    } else {
+      const double* qr = (const double*)a.Qm + (long)q * a.ldqm;
+      const double* r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r[u] = (const double*)a.Xm + (long)idx[u] * a.ldxm;
+      for (int j = 0; j < d; ++j) {
+        const double qj = qr[j];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double t1 = qj - r[u][j];
+          acc[u] = __builtin_fma(t1, t1, acc[u]);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = b + u * 64 + lane;
+      if (ok[u]) { dk[t] = __builtin_bit_cast(uint64_t, acc[u]); di[t] = idx[u]; }
+    }
+  }
+  int P = 64;
+  while (P < Kq) P <<= 1;
+  wave_sync();
+  for (int i = Kq + lane; i < P; i += 64) { dk[i] = ~0ull; di[i] = ~0u; }
+  wave_bitonic_pair(dk, di, P);
+  const long o = a.out_off[q];
+  for (int t = lane; t < k; t += 64) {
+    double dist;
+    int32_t id;
+    if (t < Kq) { dist = __builtin_bit_cast(double, dk[t]); id = (int32_t)di[t]; }
+    else { dist = __builtin_inf(); id = -1; }
+    if (a.out_f64) ((double*)a.out_dist)[o + t] = dist;
+    else ((float*)a.out_dist)[o + t] = (float)dist;
+    a.out_idx[o + t] = id;
+  }
+}
+
 // ------------------------------------------------------------------------------------------------
 // fills
 // ------------------------------------------------------------------------------------------------
@@ -808,7 +980,13 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
 template <int PREC, int QB, int DC>
 static hipError_t scan_one(const ScanArgs& a, int groups, hipStream_t stream) {
   if (groups % 4) return hipErrorInvalidValue;  // a workgroup runs 4 query groups
-  hipLaunchKernelGGL((scan_kernel<PREC, QB, DC>), dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
+  if (a.xcd_swizzle) {
+    if (a.wg_rows != groups / 4) return hipErrorInvalidValue;
+    const long total = (long)a.wg_rows * ((a.slices + 7) / 8 * 8);
+    hipLaunchKernelGGL((scan_kernel<PREC, QB, DC>), dim3((unsigned)total), dim3(256), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((scan_kernel<PREC, QB, DC>), dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
+  }
   return hipGetLastError();
 }
 
@@ -837,14 +1015,17 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
 hipError_t launch_merge(const MergeArgs& a, hipStream_t stream) {
   if (a.n <= 0) return hipSuccess;
   if (1 + 2 * a.slices > MERGE_MAX_SRC) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(merge_kernel, dim3(a.n), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(merge_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
 hipError_t launch_final(const FinalArgs& a, hipStream_t stream) {
   if (a.n <= 0) return hipSuccess;
   if (a.carry_cap > FINAL_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(final_kernel, dim3(a.n), dim3(256), 0, stream, a);
+  if (a.carry_cap <= FW_MAX)
+    hipLaunchKernelGGL(final_wave_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL(final_block_kernel, dim3(a.n), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
