@@ -102,6 +102,26 @@ struct Engine {
     return KNNS_OK;
   }
 
+  void* pin = nullptr;
+  size_t pin_cap = 0;
+  // grow-only pinned host staging (reused only after the previous search synchronised)
+  knns_status pinned(size_t bytes, void** out) {
+    if (pin_cap < bytes) {
+      if (pin) (void)hipHostFree(pin);
+      pin = nullptr;
+      pin_cap = 0;
+      const size_t want = bytes + bytes / 4 + 4096;
+      if (hipHostMalloc(&pin, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        pin = nullptr;
+        return fail(KNNS_ERR_NOMEM, "pinned host allocation of %zu bytes failed", want);
+      }
+      pin_cap = want;
+    }
+    *out = pin;
+    return KNNS_OK;
+  }
+
   void release() {
     for (int s = 0; s < W_NSLOTS; ++s) {
       if (buf[s]) (void)hipFree(buf[s]);
@@ -166,17 +186,31 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   P.n_pad = P.groups * qw;
   P.kmax = kmax;
   P.cmax = cmax;
-  // scan workgroups: 4 query groups each, 2 resident per CU; aim for ~4 full waves of workgroups
-  // so the last, partial one is a small tail
+  // scan workgroups: 4 query groups each, 2 resident per CU. Choose the slice count so that the
+  // workgroups of a large round fill whole waves of resident workgroups (a last wave that is
+  // mostly empty leaves the chip idle): the smallest S with >= 4 waves and >= 95% of the last
+  // wave used, else the S with the best fill.
   const long wg_rows = P.groups / 4;
-  const long target_wgs = (long)cus * 2 * 4;
-  int smax = (int)std::min<long>(128, std::max<long>(1, (target_wgs + wg_rows - 1) / wg_rows));
+  const long cap = (long)cus * 2;
+  int smax = 1;
+  {
+    double best = -1.0;
+    for (int S = 1; S <= 128; ++S) {
+      const long wgs = wg_rows * S;
+      const long waves = (wgs + cap - 1) / cap;
+      const double fill = (double)wgs / (double)(waves * cap);
+      if (waves >= 4 && fill >= 0.95) { smax = S; best = 2.0; break; }
+      const double score = fill * std::min<double>(1.0, (double)wgs / (4.0 * cap));
+      if (score > best) { best = score; smax = S; }
+    }
+  }
   if (slices_req > 0) smax = std::min(128, slices_req);
   // growth factor of the corpus prefix between threshold rounds: a fixed ratio if requested,
-  // else x4 while the prefix is under m/8 (cheap rounds, fewer merges) and x2 after (the large
+  // else x4 while the prefix is under m/16 (cheap rounds, fewer merges) and x2 after (the large
   // rounds, where a tighter threshold saves filter work)
+  const long gr = scan_stage_rows(P.prec_kernel, P.dp, P.dc);  // boundary granularity (rows)
   long b = 0;
-  long n0 = std::min<long>(m, round_up(std::max<long>(2L * cmax, 64), 64));
+  long n0 = std::min<long>(m, round_up(std::max<long>(2L * cmax, gr), gr));
   long e = n0;
   int smax_used = 1;
   while (b < m) {
@@ -185,13 +219,13 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
     r.end = e;
     const long rows = e - b;
     int s = (int)std::min<long>(smax, std::max<long>(1, rows / 256));
-    r.slice_rows = round_up((rows + s - 1) / s, 64);
+    r.slice_rows = round_up((rows + s - 1) / s, gr);
     r.slices = (int)((rows + r.slice_rows - 1) / r.slice_rows);
     smax_used = std::max(smax_used, r.slices);
     P.rounds.push_back(r);
     b = e;
-    const long ratio = ratio_req >= 2 ? ratio_req : (e * 8 < m ? 4 : 2);
-    e = std::min<long>(m, round_up(e * ratio, 64));
+    const long ratio = ratio_req >= 2 ? ratio_req : (e * 16 < m ? 4 : 2);
+    e = std::min<long>(m, round_up(e * ratio, gr));
   }
   P.slices_max = smax_used;
   P.sub_cap = (int)round_up(std::max<long>(cmax + 96, 256), 64);
@@ -237,23 +271,36 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   const long m = c->m;
 
   // per-query k and shortlist
-  std::vector<int> kq(n), kc(n);
-  std::vector<long> off(n);
-  long total_out = 0;
-  int kmax = 0, cmax = 0;
+  const bool uniform = kq_host == nullptr;
   const long m_eff = excl ? std::max<long>(m - 1, 1) : m;
-  for (long i = 0; i < n; ++i) {
-    const int k = kq_host ? kq_host[i] : kuni;
-    if (k < 1 || k > 1024) return fail(KNNS_ERR_INVALID, "k=%d outside [1, 1024]", k);
+  auto shortlist_of = [&](int k) {
     int cl = p->shortlist > 0 ? std::max(p->shortlist, k) : default_shortlist(prec, k);
     cl = (int)std::min<long>(cl, m_eff);
-    cl = std::min(cl, 2048);
-    kq[i] = k;
-    kc[i] = std::max(cl, 1);
-    off[i] = total_out;
-    total_out += k;
-    kmax = std::max(kmax, k);
-    cmax = std::max(cmax, kc[i]);
+    return std::max(std::min(cl, 2048), 1);
+  };
+  std::vector<int> kq, kc;
+  std::vector<long> off;
+  long total_out = 0;
+  int kmax = 0, cmax = 0;
+  if (uniform) {
+    if (kuni < 1 || kuni > 1024) return fail(KNNS_ERR_INVALID, "k=%d outside [1, 1024]", kuni);
+    kmax = kuni;
+    cmax = shortlist_of(kuni);
+    total_out = n * (long)kuni;
+  } else {
+    kq.resize(n);
+    kc.resize(n);
+    off.resize(n);
+    for (long i = 0; i < n; ++i) {
+      const int k = kq_host[i];
+      if (k < 1 || k > 1024) return fail(KNNS_ERR_INVALID, "k=%d outside [1, 1024]", k);
+      kq[i] = k;
+      kc[i] = shortlist_of(k);
+      off[i] = total_out;
+      total_out += k;
+      kmax = std::max(kmax, k);
+      cmax = std::max(cmax, kc[i]);
+    }
   }
 
   Plan P = make_plan(m, c->d, n, kmax, cmax, prec, E.cus, p->ratio, p->slices);
@@ -320,27 +367,23 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   pa.is_query = 1;
   KNNS_HIP(launch_pack(pa, st));
 
-  // per-query control arrays
-  {
-    std::vector<uint32_t> thr_h(n_pad, ORD_NEG_INF);
-    for (long i = 0; i < n; ++i) thr_h[i] = ORD_POS_INF;
-    std::vector<int> kc_h(n_pad, 1);
-    for (long i = 0; i < n; ++i) kc_h[i] = kc[i];
-    KNNS_HIP(hipMemcpyAsync(thr, thr_h.data(), n_pad * 4, hipMemcpyHostToDevice, st));
-    KNNS_HIP(hipMemcpyAsync(kcd, kc_h.data(), n_pad * 4, hipMemcpyHostToDevice, st));
-    KNNS_HIP(hipMemcpyAsync(kqd, kq.data(), n * 4, hipMemcpyHostToDevice, st));
-    KNNS_HIP(hipMemcpyAsync(offd, off.data(), n * 8, hipMemcpyHostToDevice, st));
-    if (excl) {
-      std::vector<int> self_h(n_pad, -1);
-      for (long i = 0; i < n; ++i) {
-        const long r = p->self_offset + i;
-        self_h[i] = (r >= 0 && r < m) ? (int)r : -1;
-      }
-      KNNS_HIP(hipMemcpyAsync(selfd, self_h.data(), n_pad * 4, hipMemcpyHostToDevice, st));
-    }
-    KNNS_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
+  // per-query control arrays: built on the device (uniform k), per-query values staged through
+  // pinned host memory otherwise; either way no synchronisation inside the search
+  KNNS_HIP(launch_init_queries((uint32_t*)thr, (int*)kcd, (int*)kqd, (long*)offd, (int*)selfd,
+                               (int*)cc0, n, n_pad, uniform ? kuni : 1, cmax, p->self_offset, m, st));
+  if (!uniform) {
+    void* pin;
+    if ((sst = E.pinned(n * 16, &pin)) != KNNS_OK) return sst;
+    int* pk = (int*)pin;
+    int* pc = pk + n;
+    long* po = (long*)(pc + n + (n & 1));
+    std::memcpy(pk, kq.data(), n * 4);
+    std::memcpy(pc, kc.data(), n * 4);
+    std::memcpy(po, off.data(), n * 8);
+    KNNS_HIP(hipMemcpyAsync(kqd, pk, n * 4, hipMemcpyHostToDevice, st));
+    KNNS_HIP(hipMemcpyAsync(kcd, pc, n * 4, hipMemcpyHostToDevice, st));
+    KNNS_HIP(hipMemcpyAsync(offd, po, n * 8, hipMemcpyHostToDevice, st));
   }
-  KNNS_HIP(launch_fill_i32((int*)cc0, 0, n, st));
   KNNS_HIP(launch_fill_i32((int*)flags, 0, 16, st));
 
   const void* xop = P.prec_kernel == PREC_BF16 ? (const void*)c->x16 : (const void*)c->x32;
@@ -499,7 +542,7 @@ knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, lo
   c->dtype = dtype;
   c->flags = flags;
   dims_for(d, &c->dp, &c->dc);
-  c->rows_pad = round_up(m, 64);
+  c->rows_pad = round_up(m, 128);  // whole scan stages (64 or 128 rows) never leave the padding
   c->ld32 = c->dp + 4;
   c->ld16 = c->dp + 16;
   auto cleanup = [&](knns_status st) {

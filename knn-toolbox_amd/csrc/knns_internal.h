@@ -110,10 +110,11 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
 hipError_t launch_merge(const MergeArgs& a, hipStream_t stream);
 hipError_t launch_final(const FinalArgs& a, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t stream);
+hipError_t launch_init_queries(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
+                               int* carry_cnt, long n, long n_pad, int k, int shortlist,
+                               long self_offset, long m, hipStream_t stream);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t stream);
-hipError_t launch_to_knntype(const float* dsrc_f32, const double* dsrc_f64, const int32_t* isrc,
-                             void* dist_out, void* idx_out, int out_f64, long count,
-                             hipStream_t stream);
 bool scan_supported(int prec, int qb, int dc);
+int scan_stage_rows(int prec, int dp, int dc);
 
 }  // namespace knns

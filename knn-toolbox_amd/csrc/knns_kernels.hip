@@ -259,17 +259,22 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
 // and tightens that query's threshold.
 template <int QB>
 struct ScanState {
-  float phi[QB];        // D-space threshold: admit D > phi  (phi = -threshold)
-  uint32_t thr[QB];     // ordered threshold
+  float phi[QB];        // D-space threshold: admit D > phi  (phi = -threshold, exact negation)
   int cnt[QB];          // this lane's sub-pool fill
-  long pbase[QB];       // this lane's sub-pool base
-  int selfid[QB];
+  int q0;               // first query of this wave's group
+  int sl;               // slice
+  // rarely used values are recomputed where needed instead of occupying registers
+  __device__ __forceinline__ long pbase(const ScanArgs& a, int qb) const {
+    const int q = q0 + qb * 32 + (int)(threadIdx.x & 31);
+    return (((long)q * a.slices_max + sl) * 2 + (int)((threadIdx.x & 63) >> 5)) * a.sub_cap;
+  }
+  __device__ __forceinline__ uint32_t thr(int qb) const { return f2ord(-phi[qb]); }
 };
 
 template <int QB>
 __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& st, int g, int sl,
                                           uint32_t* lds) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int lim = a.sub_cap - 32;
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
@@ -310,9 +315,14 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
       if (lane == c) { st.cnt[qb] = k0; }
       if (lane == c + 32) { st.cnt[qb] = k1; }
       if ((lane & 31) == c) {
-        if (T < st.thr[qb]) { st.thr[qb] = T; st.phi[qb] = -ord2f(T); }
+        if (T < st.thr(qb)) st.phi[qb] = -ord2f(T);
       }
-      if (k0 > lim || k1 > lim) { if (lane == 0) atomicAdd(&a.flags[0], 1); }
+      if (k0 > lim || k1 > lim) {
+        // no room for another tile: stop admitting for this query and report the overflow
+        // (results of this search are then rejected by the host)
+        if ((lane & 31) == c) st.phi[qb] = __builtin_inff();
+        if (lane == 0) atomicAdd(&a.flags[0], 1);
+      }
     }
   }
 }
@@ -348,68 +358,74 @@ __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& 
                            fmaxf(fmaxf(gm[4], gm[5]), fmaxf(gm[6], gm[7])));
     const float phi = st.phi[qb];
     if (__builtin_amdgcn_ballot_w64(qm > phi) == 0) continue;
-    const int self_rel = st.selfid[qb] < 0 ? -1 : st.selfid[qb] - (int)t0;
+    int self_rel = -1;
+    if (a.self_ids) {
+      const int sid = a.self_ids[st.q0 + qb * 32 + (int)(threadIdx.x & 31)];
+      if (sid >= 0) self_rel = sid - (int)t0;
+    }
+    uint64_t* pool = a.sub + st.pbase(a, qb);
 #pragma unroll
     for (int gi = 0; gi < 8; ++gi) {
       if (__builtin_amdgcn_ballot_w64(gm[gi] > phi) == 0) continue;
-      const int cb = gi >> 2, v0 = (gi & 3) * 4;
+      if (gm[gi] > phi) {
+        // lanes holding a survivor in this group of 4 rows: append each row unconditionally at
+        // the current fill and advance the fill only for survivors (room for a whole tile is
+        // guaranteed before the tile starts, so the write stays inside the sub-pool)
+        const int cb = gi >> 2, v0 = (gi & 3) * 4;
+        int cnt = st.cnt[qb];
 #pragma unroll
-      for (int v = v0; v < v0 + 4; ++v) {
-        const float D = Dv[cb][v];
-        if (D > phi) {
+        for (int v = v0; v < v0 + 4; ++v) {
+          const float D = Dv[cb][v];
           const int rel = cb * 32 + 8 * (v >> 2) + (v & 3) + h4;
-          if (rel != self_rel) {
-            const float sk = -D + 0.0f;
-            const uint64_t e = ((uint64_t)f2ord(sk) << 32) | (uint32_t)(tb + (uint32_t)rel);
-            if (st.cnt[qb] < a.sub_cap) {
-              a.sub[st.pbase[qb] + st.cnt[qb]] = e;
-              st.cnt[qb] += 1;
-            } else {
-              atomicAdd(&a.flags[0], 1);
-            }
-          }
+          const float sk = -D + 0.0f;
+          pool[cnt] = ((uint64_t)f2ord(sk) << 32) | (uint32_t)(tb + (uint32_t)rel);
+          cnt += (D > phi && rel != self_rel) ? 1 : 0;
         }
+        st.cnt[qb] = cnt;
       }
     }
   }
 }
 
-// LDS tile of 64 corpus rows x one chunk of DC dims (+ the augmentation columns). The row stride
-// is padded to an odd multiple of 4 dwords so the 16-byte MFMA operand reads of 8 (32 banks) or
-// 16 (64 banks) consecutive rows hit distinct banks.
-template <int PREC, int DC>
+// LDS stage: SR corpus rows x one chunk of DC dims + the augmentation. A BF16 row keeps only the
+// half-0 augmentation (8 elements: the 3-term -||r~||^2/2 split); half-1 lanes read the same
+// values and multiply them by the query's zero half-1 augmentation. The row stride is padded to an
+// odd multiple of 4 dwords so the 16-byte MFMA operand reads of 8 (32 banks) or 16 (64 banks)
+// consecutive rows hit distinct banks.
+template <int PREC, int DC, int SR>
 struct TileGeom {
   static constexpr int ESZ = PREC == PREC_F32 ? 4 : 2;
-  static constexpr int AUG = PREC == PREC_F32 ? 4 : 16;
+  static constexpr int AUG = PREC == PREC_F32 ? 4 : 8;
   static constexpr int VEC = 16 / ESZ;
   static constexpr int ROW_EL = DC + AUG;
   static constexpr int ROW_DW0 = ROW_EL * ESZ / 4;
   static constexpr int STRIDE_DW = ROW_DW0 + ((4 - ROW_DW0 % 8) + 8) % 8;
-  static constexpr int STRIDE_EL = STRIDE_DW * 4 / ESZ;
-  static constexpr int UNITS = ROW_EL / VEC;
+  static constexpr int SU = STRIDE_DW / 4;          // 16-byte units per LDS row
+  static constexpr int UNITS = ROW_EL / VEC;        // units holding data
   static constexpr int MAIN_UNITS = DC / VEC;
-  static constexpr int TILE_UNITS = 64 * UNITS;
-  static constexpr int LOADS = (TILE_UNITS + 255) / 256;
-  static constexpr int BUF_DW = 64 * STRIDE_DW;
+  static constexpr int BLOCKS = SR * SU / 64;       // wave-wide DMA loads per stage
+  static constexpr int WLOADS = (BLOCKS + 3) / 4;   // per wave
+  static constexpr int BUF_DW = SR * STRIDE_DW;
   static_assert(STRIDE_DW % 8 == 4, "stride must be an odd multiple of 4 dwords");
+  static_assert(SR * SU % 64 == 0, "stage must be whole wave loads");
 };
 
 // One workgroup = 4 waves = 4 query groups (32*QB queries each) sharing one corpus slice. The
-// slice is streamed as (tile, chunk) stages through two LDS buffers: while the waves run the MFMAs
-// of stage s, the global loads of stage s+1 are in flight into registers.
-template <int PREC, int QB, int DC>
+// slice is streamed in stages of SR rows x one DC chunk through two LDS buffers filled by DMA: the
+// loads of stage s+1 are in flight while the waves run the MFMAs of stage s. A BF16 stage of 128
+// rows is processed as two 64-row sub-tiles, halving barriers and waits per MFMA (single-chunk
+// only: a sub-tile's accumulators do not outlive the stage).
+template <int PREC, int QB, int DC, int SR>
 __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
-  using G = TileGeom<PREC, DC>;
-  // BF16 tiles compute in ~1.2k cycles at peak, shorter than a DMA round trip: keep two stages in
-  // flight. FP32 tiles take 8x longer; one stage of lookahead hides the latency.
-  constexpr int NBUF = PREC == PREC_F32 ? 2 : 3;
-  constexpr int PF = NBUF - 1;   // prefetch distance in stages
+  using G = TileGeom<PREC, DC, SR>;
+  constexpr int NBUF = 2;
+  constexpr int SUB = SR / 64;
   __shared__ __attribute__((aligned(16))) uint32_t tile[NBUF][G::BUF_DW];
   __shared__ __attribute__((aligned(16))) uint32_t kth_lds[4][256 + 32];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int col = lane & 31, h = lane >> 5;
-  // Workgroups are dispatched round-robin over the 8 XCDs (linear id % 8). With a_swizzle the
+  // Workgroups are dispatched round-robin over the 8 XCDs (linear id % 8). With xcd_swizzle the
   // linear id is mapped so that every workgroup of slice sl runs on XCD sl % 8: the query rows of a
   // slice then share that XCD's L2 instead of every XCD fetching every slice.
   int wgrow, sl;
@@ -429,17 +445,16 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   if (c1 > a.row_end) c1 = a.row_end;
 
   ScanState<QB> st;
+  st.q0 = g * QB * 32;
+  st.sl = sl;
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
     const int q = (g * QB + qb) * 32 + col;
-    st.thr[qb] = a.thr[q];
-    st.phi[qb] = -ord2f(st.thr[qb]);
+    st.phi[qb] = -ord2f(a.thr[q]);
     st.cnt[qb] = 0;
-    st.pbase[qb] = (((long)q * a.slices_max + sl) * 2 + h) * a.sub_cap;
-    st.selfid[qb] = a.self_ids ? a.self_ids[q] : -1;
   }
   const int nch = a.dp / DC;
-  const long ntiles = c0 < c1 ? (c1 - c0 + 63) / 64 : 0;
+  const long ntiles = c0 < c1 ? (c1 - c0 + SR - 1) / SR : 0;
   const long nstage = ntiles * nch;
 
   typedef typename std::conditional<PREC == PREC_F32, float, uint16_t>::type T;
@@ -447,31 +462,30 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   const T* Q = (const T*)a.Qp;
 
   // ---- stage loader: global -> LDS by DMA (global_load_lds_dwordx4) ----
-  // The buffer is a dense array of 16-byte units, SU per row (the row padding is one unit). Load
-  // instruction l of wave wv fills units [(wv + 4l)*64, +64): lane -> (row, unit) -> global source.
-  // Padding units re-read the row's first unit.
-  constexpr int SU = G::STRIDE_DW / 4;
-  constexpr int WLOADS = (SU + 3) / 4;
+  // The buffer is a dense array of 16-byte units, SU per row. Load l of wave wv fills units
+  // [(wv + 4l)*64, +64): lane -> (row, unit) -> global source; padding units re-read the row start.
+  // Blocks past the end repeat the last block (same source, destination and bytes) so every wave
+  // issues exactly WLOADS loads per stage.
+  const uint32_t ldx32 = (uint32_t)a.ldx;   // < 2^16 elements: in-stage offsets fit 32 bits
   auto load_stage = [&](long s, int buf) {
-    const long t0 = c0 + (s / nch) * 64;
+    const long t0 = c0 + (s / nch) * SR;
     const int c = (int)(s % nch);
+    const T* stage = X + t0 * a.ldx;
+    // recompute the per-lane offsets at every stage instead of keeping WLOADS of them live across
+    // the loop (the empty asm hides that `ln` equals the loop-invariant lane id)
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
 #pragma unroll
-    for (int l = 0; l < WLOADS; ++l) {
-      // blocks past the end repeat the last block (same source, same destination, same bytes) so
-      // every wave has exactly WLOADS loads in flight per stage and vmcnt waits can count stages
-      const int blk = wv + 4 * l < SU ? wv + 4 * l : SU - 1;
-      {
-        const int U = blk * 64 + lane;
-        const int r = U / SU, u = U % SU;
-        const T* row = X + (t0 + r) * a.ldx;
-        const T* src = u < G::MAIN_UNITS ? row + (long)c * DC + u * G::VEC
-                     : (u < G::UNITS ? row + a.dp + (u - G::MAIN_UNITS) * G::VEC : row);
-        __builtin_amdgcn_global_load_lds((const void*)src,
-            (void __attribute__((address_space(3)))*)&tile[buf][blk * 256], 16, 0, 0);
-      }
+    for (int l = 0; l < G::WLOADS; ++l) {
+      const int blk = wv + 4 * l < G::BLOCKS ? wv + 4 * l : G::BLOCKS - 1;
+      const int U = blk * 64 + ln;
+      const uint32_t r = (uint32_t)(U / G::SU), u = (uint32_t)(U % G::SU);
+      const uint32_t off = r * ldx32 + (u < G::MAIN_UNITS ? (uint32_t)(c * DC) + u * G::VEC
+                                      : (u < G::UNITS ? (uint32_t)a.dp + (u - G::MAIN_UNITS) * G::VEC : 0u));
+      __builtin_amdgcn_global_load_lds((const void*)(stage + off),
+          (void __attribute__((address_space(3)))*)&tile[buf][blk * 256], 16, 0, 0);
     }
   };
-  static_assert(G::STRIDE_DW % 4 == 0, "rows are whole 16-byte units");
 
   // ---- per-wave query operand (B) ----
   constexpr int NB = PREC == PREC_F32 ? DC / 2 : DC / 16;
@@ -493,93 +507,97 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
       }
     }
   };
-  BT baug[QB];
-#pragma unroll
-  for (int qb = 0; qb < QB; ++qb) {
-    const T* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + a.dp;
-    if constexpr (PREC == PREC_F32) baug[qb] = qp[h];
-    else baug[qb] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 8 * h));
+  // the query side of the augmentation is the same for every query: ones against the norm terms
+  // (FP32: hi and lo in the two lane halves; BF16: three terms in half 0, zeros in half 1)
+  BT baug;
+  if constexpr (PREC == PREC_F32) {
+    baug = 1.0f;
+  } else {
+    const uint32_t one = 0x3f80u, w01 = h ? 0u : (one | (one << 16)), w2 = h ? 0u : one;
+    baug = __builtin_bit_cast(bf16x8, (u32x4){w01, w2, 0u, 0u});
   }
   if (nch == 1) loadB(0);
 
-#pragma unroll
-  for (int p = 0; p < PF; ++p)
-    if (p < nstage) load_stage(p, p);
+  if (nstage > 0) load_stage(0, 0);
   f32x16 acc[QB][2];
   for (long s = 0; s < nstage; ++s) {
-    // stage s must have landed; the loads of the younger stages may stay in flight (VMEM
-    // operations retire in issue order, later stores only make this wait longer)
-    const long younger = nstage - 1 - s < PF - 1 ? nstage - 1 - s : PF - 1;
-    if (younger >= 1) wait_vmcnt<WLOADS>();
-    else __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();   // every wave's DMA for stage s is visible; buffer of stage s-1 is free
-    const int buf = (int)(s % NBUF);
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's DMA for stage s has landed
+    __syncthreads();                 // ... every wave's; and nobody still reads the other buffer
+    const int buf = (int)(s & 1);
     const int c = (int)(s % nch);
-    const long t0 = c0 + (s / nch) * 64;
-    if (s + PF < nstage) load_stage(s + PF, (int)((s + PF) % NBUF));
+    const long t0s = c0 + (s / nch) * SR;
+    if (s + 1 < nstage) load_stage(s + 1, buf ^ 1);
     if (nch > 1) loadB(c);
-    if (c == 0) {
 #pragma unroll
-      for (int qb = 0; qb < QB; ++qb)
+    for (int sub = 0; sub < SUB; ++sub) {
+      const long t0 = t0s + 64 * sub;
+      if (SUB > 1 && t0 >= c1) break;
+      // one sub-tile's accumulators at a time (register budget of 2 waves/SIMD)
+      __builtin_amdgcn_sched_barrier(0);
+      if (c == 0) {
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) acc[qb][cb] = (f32x16){0};
-    }
-    const uint32_t* r0 = &tile[buf][col * G::STRIDE_DW];
-    const uint32_t* r1 = &tile[buf][(32 + col) * G::STRIDE_DW];
-    const int hoff = h * (DC / 2) * G::ESZ / 4;  // dword offset of this lane half
-    if constexpr (PREC == PREC_F32) {
+        for (int qb = 0; qb < QB; ++qb)
 #pragma unroll
-      for (int j = 0; j < DC / 2; j += 4) {
-        const f32x4 a0 = __builtin_bit_cast(f32x4, *(const u32x4*)(r0 + hoff + j));
-        const f32x4 a1 = __builtin_bit_cast(f32x4, *(const u32x4*)(r1 + hoff + j));
+          for (int cb = 0; cb < 2; ++cb) acc[qb][cb] = (f32x16){0};
+      }
+      const uint32_t* r0 = &tile[buf][(64 * sub + col) * G::STRIDE_DW];
+      const uint32_t* r1 = &tile[buf][(64 * sub + 32 + col) * G::STRIDE_DW];
+      const int hoff = h * (DC / 2) * G::ESZ / 4;  // dword offset of this lane half
+      if constexpr (PREC == PREC_F32) {
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
+        for (int j = 0; j < DC / 2; j += 4) {
+          const f32x4 a0 = __builtin_bit_cast(f32x4, *(const u32x4*)(r0 + hoff + j));
+          const f32x4 a1 = __builtin_bit_cast(f32x4, *(const u32x4*)(r1 + hoff + j));
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+#pragma unroll
+            for (int qb = 0; qb < QB; ++qb) {
+              acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[jj], bq[qb][j + jj], acc[qb][0], 0, 0, 0);
+              acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[jj], bq[qb][j + jj], acc[qb][1], 0, 0, 0);
+            }
+          }
+        }
+        if (c == nch - 1) {
+          const float g0 = __builtin_bit_cast(float, r0[DC + h]);
+          const float g1 = __builtin_bit_cast(float, r1[DC + h]);
 #pragma unroll
           for (int qb = 0; qb < QB; ++qb) {
-            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[jj], bq[qb][j + jj], acc[qb][0], 0, 0, 0);
-            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[jj], bq[qb][j + jj], acc[qb][1], 0, 0, 0);
+            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(g0, baug, acc[qb][0], 0, 0, 0);
+            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(g1, baug, acc[qb][1], 0, 0, 0);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < DC / 16; ++j) {
+          const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r0 + hoff + 4 * j));
+          const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r1 + hoff + 4 * j));
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb) {
+            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[qb][j], acc[qb][0], 0, 0, 0);
+            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[qb][j], acc[qb][1], 0, 0, 0);
+          }
+          // keep at most two steps of operands in flight (register budget of 2 waves/SIMD)
+          if ((j & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+        }
+        if (c == nch - 1) {
+          // both lane halves read the half-0 augmentation; half 1 multiplies it by zeros
+          const bf16x8 g0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r0 + DC / 2));
+          const bf16x8 g1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r1 + DC / 2));
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb) {
+            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g0, baug, acc[qb][0], 0, 0, 0);
+            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g1, baug, acc[qb][1], 0, 0, 0);
           }
         }
       }
       if (c == nch - 1) {
-        const float g0 = __builtin_bit_cast(float, r0[DC + h]);
-        const float g1 = __builtin_bit_cast(float, r1[DC + h]);
+        if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
+        else scan_epilogue<QB, false>(a, st, acc, t0, c1);
+        bool trig = false;
 #pragma unroll
-        for (int qb = 0; qb < QB; ++qb) {
-          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(g0, baug[qb], acc[qb][0], 0, 0, 0);
-          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(g1, baug[qb], acc[qb][1], 0, 0, 0);
-        }
+        for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 32;
+        if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv]);
       }
-    } else {
-#pragma unroll
-      for (int j = 0; j < DC / 16; ++j) {
-        const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r0 + hoff + 4 * j));
-        const bf16x8 a1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r1 + hoff + 4 * j));
-#pragma unroll
-        for (int qb = 0; qb < QB; ++qb) {
-          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bq[qb][j], acc[qb][0], 0, 0, 0);
-          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bq[qb][j], acc[qb][1], 0, 0, 0);
-        }
-        // keep at most two steps of operands in flight (register budget of 2 waves/SIMD)
-        if ((j & 1) == 1) __builtin_amdgcn_sched_barrier(0);
-      }
-      if (c == nch - 1) {
-        const bf16x8 g0 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r0 + DC / 2 + 4 * h));
-        const bf16x8 g1 = __builtin_bit_cast(bf16x8, *(const u32x4*)(r1 + DC / 2 + 4 * h));
-#pragma unroll
-        for (int qb = 0; qb < QB; ++qb) {
-          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g0, baug[qb], acc[qb][0], 0, 0, 0);
-          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(g1, baug[qb], acc[qb][1], 0, 0, 0);
-        }
-      }
-    }
-    if (c == nch - 1) {
-      if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
-      else scan_epilogue<QB, false>(a, st, acc, t0, c1);
-      bool trig = false;
-#pragma unroll
-      for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 32;
-      if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv]);
     }
   }
   // publish fills
@@ -956,6 +974,37 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// per-query control arrays for a uniform-k search (no host upload): thresholds (+inf for real
+// queries, -inf for padding queries so nothing survives), shortlist, k, output offset, self id
+// ------------------------------------------------------------------------------------------------
+__global__ void init_queries_kernel(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
+                                    int* carry_cnt, long n, long n_pad, int k, int shortlist,
+                                    long self_offset, long m) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pad) return;
+  thr[i] = i < n ? ORD_POS_INF : ORD_NEG_INF;
+  kc[i] = shortlist;
+  if (self_ids) {
+    const long r = self_offset + i;
+    self_ids[i] = (i < n && r >= 0 && r < m) ? (int)r : -1;
+  }
+  if (i < n) {
+    kq[i] = k;
+    off[i] = i * (long)k;
+    carry_cnt[i] = 0;
+  }
+}
+
+hipError_t launch_init_queries(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
+                               int* carry_cnt, long n, long n_pad, int k, int shortlist,
+                               long self_offset, long m, hipStream_t stream) {
+  if (n_pad <= 0) return hipSuccess;
+  hipLaunchKernelGGL(init_queries_kernel, dim3((unsigned)((n_pad + 255) / 256)), dim3(256), 0, stream,
+                     thr, kc, kq, off, self_ids, carry_cnt, n, n_pad, k, shortlist, self_offset, m);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
 // fills
 // ------------------------------------------------------------------------------------------------
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, long n) {
@@ -977,15 +1026,15 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-template <int PREC, int QB, int DC>
+template <int PREC, int QB, int DC, int SR>
 static hipError_t scan_one(const ScanArgs& a, int groups, hipStream_t stream) {
   if (groups % 4) return hipErrorInvalidValue;  // a workgroup runs 4 query groups
   if (a.xcd_swizzle) {
     if (a.wg_rows != groups / 4) return hipErrorInvalidValue;
     const long total = (long)a.wg_rows * ((a.slices + 7) / 8 * 8);
-    hipLaunchKernelGGL((scan_kernel<PREC, QB, DC>), dim3((unsigned)total), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((scan_kernel<PREC, QB, DC, SR>), dim3((unsigned)total), dim3(256), 0, stream, a);
   } else {
-    hipLaunchKernelGGL((scan_kernel<PREC, QB, DC>), dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL((scan_kernel<PREC, QB, DC, SR>), dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
   }
   return hipGetLastError();
 }
@@ -993,21 +1042,26 @@ static hipError_t scan_one(const ScanArgs& a, int groups, hipStream_t stream) {
 bool scan_supported(int prec, int qb, int dc) {
   if (prec != PREC_F32 && prec != PREC_BF16) return false;
   if (qb != 1 && qb != 2) return false;
-  if (prec == PREC_F32) return dc == 32 || dc == 64 || dc == 96 || dc == 128;
   return dc == 32 || dc == 64 || dc == 96 || dc == 128;
 }
 
+// Row granularity of the stages: slice and round boundaries must be multiples of it, and the
+// corpus must be padded to it, so a stage never reads past the padded corpus.
+int scan_stage_rows(int prec, int dp, int dc) {
+  return (prec == PREC_BF16 && dp == dc) ? 128 : 64;
+}
+
 hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, hipStream_t stream) {
-#define KNNS_SCAN_CASE(P, B, D) \
-  if (prec == P && qb == B && dc == D) return scan_one<P, B, D>(a, groups, stream);
-  KNNS_SCAN_CASE(PREC_F32, 1, 32) KNNS_SCAN_CASE(PREC_F32, 1, 64)
-  KNNS_SCAN_CASE(PREC_F32, 1, 96) KNNS_SCAN_CASE(PREC_F32, 1, 128)
-  KNNS_SCAN_CASE(PREC_F32, 2, 32) KNNS_SCAN_CASE(PREC_F32, 2, 64)
-  KNNS_SCAN_CASE(PREC_F32, 2, 96) KNNS_SCAN_CASE(PREC_F32, 2, 128)
-  KNNS_SCAN_CASE(PREC_BF16, 1, 32) KNNS_SCAN_CASE(PREC_BF16, 1, 64)
-  KNNS_SCAN_CASE(PREC_BF16, 1, 96) KNNS_SCAN_CASE(PREC_BF16, 1, 128)
-  KNNS_SCAN_CASE(PREC_BF16, 2, 32) KNNS_SCAN_CASE(PREC_BF16, 2, 64)
-  KNNS_SCAN_CASE(PREC_BF16, 2, 96) KNNS_SCAN_CASE(PREC_BF16, 2, 128)
+  const int sr = scan_stage_rows(prec, a.dp, dc);
+  if ((a.row_begin % sr) || (a.slice_rows % sr)) return hipErrorInvalidValue;
+#define KNNS_SCAN_CASE(P, B, D, S) \
+  if (prec == P && qb == B && dc == D && sr == S) return scan_one<P, B, D, S>(a, groups, stream);
+#define KNNS_SCAN_DC(P, B, S) \
+  KNNS_SCAN_CASE(P, B, 32, S) KNNS_SCAN_CASE(P, B, 64, S) KNNS_SCAN_CASE(P, B, 96, S) KNNS_SCAN_CASE(P, B, 128, S)
+  KNNS_SCAN_DC(PREC_F32, 1, 64) KNNS_SCAN_DC(PREC_F32, 2, 64)
+  KNNS_SCAN_DC(PREC_BF16, 1, 64) KNNS_SCAN_DC(PREC_BF16, 2, 64)
+  KNNS_SCAN_DC(PREC_BF16, 1, 128) KNNS_SCAN_DC(PREC_BF16, 2, 128)
+#undef KNNS_SCAN_DC
 #undef KNNS_SCAN_CASE
   return hipErrorInvalidValue;
 }

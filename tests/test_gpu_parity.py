@@ -141,3 +141,24 @@ def test_mixed_k_batch_equals_individual(gpu, knns_mod, oracle_mod):
     D, I = oracle_mod.exact_topk(Q, R, 100)
     for i in range(200):
         assert np.array_equal(res[i][1], I[i, :kq[i]])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_adversarial_order_forces_in_scan_compaction(gpu, knns_mod, oracle_mod, prec):
+    """Corpus rows get closer to every query as the row index grows, so every row beats the
+    running threshold and the per-lane sub-pools overflow inside a round: the in-kernel exact
+    K-th compaction must run and keep the right candidates."""
+    rng = np.random.default_rng(11)
+    m, d, n, k = 120_000, 64, 70, 20
+    # row j sits at distance (m - j) from the origin along axis 0; queries sit at the origin
+    R = np.zeros((m, d), dtype=np.float32)
+    R[:, 0] = np.arange(m, 0, -1, dtype=np.float32)
+    Q = (rng.standard_normal((n, d)) * 1e-3).astype(np.float32)
+    _print_case("adversarial", m=m, n=n, d=d, k=k, prec=prec)
+    D, I = oracle_mod.exact_topk(Q, R, k)
+    with knns_mod.Corpus(R, bf16=(prec == "bf16")) as c:
+        dd, ii = c.search(Q, k, precision=prec, slices=2)   # few slices: sub-pools must overflow
+        st = knns_mod.last_stats()
+    assert st["slow_compactions"] > 0
+    assert np.array_equal(ii.astype(np.int64), I.astype(np.int64))
+    assert np.array_equal(dd, D.astype(np.float32))
