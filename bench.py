@@ -48,6 +48,9 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-recall", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 256 MiB device copy (elementwise read+write) three times; "
+                         "its FETCH_SIZE/WRITE_SIZE counters calibrate the HBM-traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
@@ -77,6 +80,14 @@ def main() -> int:
     def run(prec: str):
         corpus.search_device(Qd, n, k, Dd, Id, precision=prec, shortlist=args.shortlist)
         return knns.last_stats()
+
+    if args.copy_calib:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---- BF16 shortlist + exact rerank (value) ----
     for _ in range(args.warmup):

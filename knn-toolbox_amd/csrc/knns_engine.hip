@@ -440,6 +440,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     ma.n = (int)n;
     ma.bitonic = bitonic;
     ma.flags = (int*)flags;
+    ma.self_ids = (const int*)selfd;
     const size_t e2 = ev++;
     KNNS_HIP(launch_merge(ma, st));
     KNNS_HIP(hipEventRecord(E.event(e2), st));

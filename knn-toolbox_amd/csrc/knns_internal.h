@@ -67,6 +67,7 @@ struct MergeArgs {
   int n;                 // real queries
   int bitonic;           // 1: exact truncated sort (keeps exactly K); 0: histogram bound
   int* flags;
+  const int* self_ids;   // optional: corpus row excluded for each query (all-kNN), -1 = none
 };
 
 struct FinalArgs {

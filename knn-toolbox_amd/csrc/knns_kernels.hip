@@ -15,6 +15,7 @@
 // MFMA precision and ||r~||^2 is the exact norm of the rounded row. The reference ranks
 // ||r||^2 - 2 q.r (include/utils.h:957-985), i.e. 2s. Using the norm of the rounded row makes s
 // exactly (||q~ - r~||^2 - ||q~||^2)/2, so the shortlist error scales with ||q - r||, not ||q||.
+#include <cstdlib>
 #include <type_traits>
 
 #include "knns_internal.h"
@@ -271,11 +272,22 @@ struct ScanState {
   __device__ __forceinline__ uint32_t thr(int qb) const { return f2ord(-phi[qb]); }
 };
 
+// Sub-pool entries are written by the scan as (bits of D) << 32 | row, D = q~.r~ - ||r~||^2/2 the
+// raw accumulator: the hot path spends no instruction on the key transform. Readers order them by
+// pool_key: (ord(s) << 32) | row with s = -D, the canonical (key, index) order of the carry.
+__device__ __forceinline__ uint64_t pool_key(uint64_t e) {
+  const float D = __builtin_bit_cast(float, (uint32_t)(e >> 32));
+  return ((uint64_t)f2ord(-D + 0.0f) << 32) | (e & 0xffffffffull);
+}
+
+// Slow path: the lanes whose sub-pool fill exceeds sub_cap - room select, per query, the exact
+// K-th entry of the query's two sub-pools (K + 1 when the self row may be among them: the merge
+// drops it later), keep the entries up to it and tighten that query's threshold.
 template <int QB>
 __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& st, int g, int sl,
-                                          uint32_t* lds) {
+                                          uint32_t* lds, int room) {
   const int lane = threadIdx.x & 63;
-  const int lim = a.sub_cap - 32;
+  const int lim = a.sub_cap - room;
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
     const uint64_t need = __builtin_amdgcn_ballot_w64(st.cnt[qb] > lim);
@@ -286,13 +298,14 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
       const int q = (g * QB + qb) * 32 + c;
       const int n0 = __builtin_amdgcn_readlane(st.cnt[qb], c);
       const int n1 = __builtin_amdgcn_readlane(st.cnt[qb], c + 32);
-      const int K = a.kc[q];
+      const int K = a.kc[q] + (a.self_ids ? 1 : 0);
       uint64_t* p0 = a.sub + (((long)q * a.slices_max + sl) * 2 + 0) * a.sub_cap;
       uint64_t* p1 = p0 + a.sub_cap;
       const int N = n0 + n1;
       if (lane == 0) atomicAdd(&a.flags[1], 1);
       if (N <= K) continue;
-      auto get = [&](int i) -> uint64_t { return i < n0 ? p0[i] : p1[i - n0]; };
+      auto raw = [&](int i) -> uint64_t { return i < n0 ? p0[i] : p1[i - n0]; };
+      auto get = [&](int i) -> uint64_t { return pool_key(raw(i)); };
       const uint64_t E = group_kth<true>(get, N, K, lds);
       // in-place order-preserving compaction of the logical array [p0 | p1] (writes never
       // overtake reads: a chunk is read completely before any of its outputs is stored)
@@ -301,7 +314,7 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
         const int i = base + lane;
         uint64_t e = 0;
         bool keep = false;
-        if (i < N) { e = get(i); keep = e <= E; }
+        if (i < N) { e = raw(i); keep = pool_key(e) <= E; }
         const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
         const int pos = outpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32),
                                   __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
@@ -331,13 +344,14 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
 // common case must cost a few VALU ops per value and no exec-mask juggling: each lane reduces its
 // values in groups of 4 consecutive rows (v_max3 + v_max), one wave-uniform ballot per query
 // block skips the block, one per group skips the group, and only groups holding a survivor run
-// the per-value divergent append.
+// the per-value append (an address, a compare and a carry per value: the entry is the raw
+// accumulator and the row, and the self row of an all-kNN query is dropped by the merge).
 template <int QB, bool FULL>
 __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& st,
                                               const f32x16 (&acc)[QB][2], long t0, long c1) {
   const int h4 = ((threadIdx.x & 63) >> 5) * 4;
   const int lim = FULL ? 64 : (int)(c1 - t0);   // valid rows of this tile
-  const uint32_t tb = (uint32_t)t0;
+  const uint32_t tb = (uint32_t)t0 + (uint32_t)h4;
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
     float Dv[2][16];
@@ -358,28 +372,22 @@ __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& 
                            fmaxf(fmaxf(gm[4], gm[5]), fmaxf(gm[6], gm[7])));
     const float phi = st.phi[qb];
     if (__builtin_amdgcn_ballot_w64(qm > phi) == 0) continue;
-    int self_rel = -1;
-    if (a.self_ids) {
-      const int sid = a.self_ids[st.q0 + qb * 32 + (int)(threadIdx.x & 31)];
-      if (sid >= 0) self_rel = sid - (int)t0;
-    }
     uint64_t* pool = a.sub + st.pbase(a, qb);
 #pragma unroll
     for (int gi = 0; gi < 8; ++gi) {
       if (__builtin_amdgcn_ballot_w64(gm[gi] > phi) == 0) continue;
       if (gm[gi] > phi) {
         // lanes holding a survivor in this group of 4 rows: append each row unconditionally at
-        // the current fill and advance the fill only for survivors (room for a whole tile is
-        // guaranteed before the tile starts, so the write stays inside the sub-pool)
+        // the current fill and advance the fill only for survivors (room for a whole stage is
+        // guaranteed before the stage starts, so the write stays inside the sub-pool)
         const int cb = gi >> 2, v0 = (gi & 3) * 4;
         int cnt = st.cnt[qb];
 #pragma unroll
         for (int v = v0; v < v0 + 4; ++v) {
           const float D = Dv[cb][v];
-          const int rel = cb * 32 + 8 * (v >> 2) + (v & 3) + h4;
-          const float sk = -D + 0.0f;
-          pool[cnt] = ((uint64_t)f2ord(sk) << 32) | (uint32_t)(tb + (uint32_t)rel);
-          cnt += (D > phi && rel != self_rel) ? 1 : 0;
+          const uint32_t rel = (uint32_t)(cb * 32 + 8 * (v >> 2) + (v & 3));
+          pool[cnt] = ((uint64_t)__builtin_bit_cast(uint32_t, D) << 32) | (tb + rel);
+          cnt += D > phi ? 1 : 0;
         }
         st.cnt[qb] = cnt;
       }
@@ -596,11 +604,185 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
         bool trig = false;
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 32;
-        if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv]);
+        if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv], 32);
       }
     }
   }
   // publish fills
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int q = (g * QB + qb) * 32 + col;
+    a.sub_cnt[((long)q * a.slices_max + sl) * 2 + h] = st.cnt[qb];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// BF16 single-chunk scan (DP = DC in {32, 64, 128}): the headline configuration
+// ------------------------------------------------------------------------------------------------
+// Same work split and filter as scan_kernel; the stage layout is built so that the loop spends
+// almost no VALU on addressing:
+//   * an LDS row holds only the SU = DC/8 16-byte units of main dims (no padding); unit u of row
+//     r sits at position u ^ sw(r), sw(r) = (r / (16/SU)) % SU, so the 16-byte MFMA operand reads
+//     of 16 consecutive rows (8 with 32 banks) touch distinct banks;
+//   * the augmentation units (16 bytes per row) go to a separate LDS array;
+//   * a wave-wide DMA load fills 64 consecutive units = 64/SU whole rows, so with load L of wave
+//     wv being L = wv + 4l, a lane's source offset inside the stage is the same for every load
+//     and every stage (one register); loads differ only by a scalar row offset;
+//   * operand reads for step j are at (A0 ^ 16j) + immediate, one XOR per step;
+//   * the first MFMA of each accumulator is the augmentation step with a zero C operand, so no
+//     accumulator is ever cleared, and operands are read one step ahead of their MFMAs.
+template <int DC>
+struct SwzGeom {
+  static constexpr int SU = DC / 8;              // 16-byte units of main dims per row
+  static constexpr int SR = 128;                 // rows per stage
+  static constexpr int RPL = 64 / SU;            // rows per wave-wide DMA load
+  static constexpr int MAIN_LOADS = SR / RPL;    // per stage
+  static constexpr int PER_WAVE = MAIN_LOADS / 4;
+  static constexpr int MAIN_BYTES = SR * SU * 16;
+  static constexpr int AUG_BYTES = SR * 16;
+  static_assert(SU == 4 || SU == 8 || SU == 16, "single-chunk widths 32, 64, 128");
+  static_assert(MAIN_LOADS % 4 == 0, "whole loads per wave");
+};
+
+template <int QB, int DC>
+__global__ __launch_bounds__(256, 2) void scan_bf16_kernel(ScanArgs a) {
+  using G = SwzGeom<DC>;
+  constexpr int SU = G::SU;
+  constexpr int J = DC / 16;                     // MFMA k-steps per row block (one per unit pair)
+  __shared__ __attribute__((aligned(1024))) uint8_t smem[2 * G::MAIN_BYTES + 2 * G::AUG_BYTES];
+  __shared__ __attribute__((aligned(16))) uint32_t kth_lds[4][256 + 32];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar addressing
+  const int col = lane & 31, h = lane >> 5;
+  int wgrow, sl;
+  if (a.xcd_swizzle) {
+    const int id = blockIdx.x;
+    const int xcd = id & 7, local = id >> 3;
+    wgrow = local % a.wg_rows;
+    sl = xcd + 8 * (local / a.wg_rows);
+    if (sl >= a.slices) return;
+  } else {
+    wgrow = blockIdx.x;
+    sl = blockIdx.y;
+  }
+  const int g = wgrow * 4 + wv;
+  // corpus rows are int32 indices: every row offset below fits 32 bits
+  const int c0 = (int)(a.row_begin + (long)sl * a.slice_rows);
+  int c1 = c0 + (int)a.slice_rows;
+  if (c1 > (int)a.row_end) c1 = (int)a.row_end;
+
+  ScanState<QB> st;
+  st.q0 = g * QB * 32;
+  st.sl = sl;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int q = (g * QB + qb) * 32 + col;
+    st.phi[qb] = -ord2f(a.thr[q]);
+    st.cnt[qb] = 0;
+  }
+  const int nst = c0 < c1 ? (c1 - c0 + G::SR - 1) / G::SR : 0;
+  const uint8_t* X = (const uint8_t*)a.X;
+  const uint32_t ldb = (uint32_t)a.ldx * 2u;     // row stride in bytes
+
+  // ---- DMA source offsets (bytes, relative to the first row of the load) ----
+  const int swl = (4 * wv + (lane >> 4)) % SU;
+  const uint32_t dma_off = (uint32_t)(lane / SU) * ldb + (uint32_t)((lane % SU) ^ swl) * 16u;
+  const uint32_t aug_off = (uint32_t)lane * ldb + (uint32_t)a.dp * 2u;
+  typedef __attribute__((address_space(3))) uint8_t lds_u8;
+  auto issue = [&](int t0, int buf) {
+    // running scalar source and LDS pointers (the empty asm keeps them in SGPRs and stops the
+    // compiler from folding the lane offset into per-load 64-bit vector addresses)
+    const uint8_t* p = X + (size_t)(t0 + wv * G::RPL) * ldb;
+    lds_u8* l = (lds_u8*)(smem + buf * G::MAIN_BYTES + wv * 1024);
+#pragma unroll
+    for (int k = 0; k < G::PER_WAVE; ++k) {
+      asm volatile("" : "+s"(p), "+s"(l));
+      __builtin_amdgcn_global_load_lds((const void*)(p + dma_off), (void __attribute__((address_space(3)))*)l,
+                                       16, 0, 0);
+      p += (size_t)(4 * G::RPL) * ldb;
+      l += 4096;
+    }
+    if (wv < G::SR / 64) {
+      const uint8_t* pa = X + (size_t)(t0 + wv * 64) * ldb;
+      lds_u8* la = (lds_u8*)(smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES + wv * 1024);
+      asm volatile("" : "+s"(pa), "+s"(la));
+      __builtin_amdgcn_global_load_lds((const void*)(pa + aug_off), (void __attribute__((address_space(3)))*)la,
+                                       16, 0, 0);
+    }
+  };
+
+  // ---- per-wave query operand (B), resident for the whole slice ----
+  bf16x8 bq[QB][J];
+  {
+    const uint16_t* Q = (const uint16_t*)a.Qp;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      const uint16_t* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + h * (DC / 2);
+#pragma unroll
+      for (int j = 0; j < J; ++j) bq[qb][j] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 8 * j));
+    }
+  }
+  bf16x8 baug;
+  {
+    const uint32_t one = 0x3f80u, w01 = h ? 0u : (one | (one << 16)), w2 = h ? 0u : one;
+    baug = __builtin_bit_cast(bf16x8, (u32x4){w01, w2, 0u, 0u});
+  }
+  // ---- operand read offsets (bytes inside a buffer) ----
+  const int swc = (col / (16 / SU)) % SU;
+  const uint32_t A0 = (uint32_t)col * SU * 16u + (uint32_t)(((h * (SU / 2)) ^ swc) * 16);
+  const uint32_t AG = (uint32_t)col * 16u;
+
+  if (nst > 0) issue(c0, 0);
+  f32x16 acc[QB][2];
+  for (int s = 0; s < nst; ++s) {
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's DMA for stage s has landed
+    __syncthreads();                 // ... every wave's; and nobody still reads the other buffer
+    const int buf = s & 1;
+    const int t0s = c0 + s * G::SR;
+    if (s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
+    const uint8_t* mb = smem + buf * G::MAIN_BYTES;
+    const uint8_t* ab = smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES;
+#pragma unroll
+    for (int sub = 0; sub < 2; ++sub) {
+      const int t0 = t0s + 64 * sub;
+      if (t0 >= c1) break;
+      __builtin_amdgcn_sched_barrier(0);
+      const uint8_t* m0 = mb + sub * 64 * SU * 16;
+      const uint8_t* m1 = m0 + 32 * SU * 16;
+      auto rd = [&](const uint8_t* p, uint32_t off) -> bf16x8 {
+        return __builtin_bit_cast(bf16x8, *(const u32x4*)(p + off));
+      };
+      bf16x8 x0 = rd(ab + sub * 64 * 16, AG), x1 = rd(ab + sub * 64 * 16 + 32 * 16, AG);
+      bf16x8 y0 = rd(m0, A0), y1 = rd(m1, A0);
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, baug, (f32x16){0}, 0, 0, 0);
+        acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, baug, (f32x16){0}, 0, 0, 0);
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        x0 = y0; x1 = y1;
+        if (j + 1 < J) {
+          const uint32_t o = A0 ^ (uint32_t)(16 * (j + 1));
+          y0 = rd(m0, o); y1 = rd(m1, o);
+        }
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, bq[qb][j], acc[qb][0], 0, 0, 0);
+          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, bq[qb][j], acc[qb][1], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
+      else scan_epilogue<QB, false>(a, st, acc, t0, c1);
+    }
+    // room for the next stage: at most 64 appends per lane and query block
+    bool trig = false;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 64;
+    if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv], 64);
+  }
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
     const int q = (g * QB + qb) * 32 + col;
@@ -668,6 +850,9 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
   const int K = a.kc[q];
   const uint64_t* cin = a.carry_in + (long)q * a.carry_cap;
   const uint64_t* sub = a.sub + subq * a.sub_cap;
+  // the self row of an all-kNN query (appended by the scan like any row) maps to the largest key
+  // and is never kept: the carry never holds it
+  const uint32_t self = a.self_ids ? (uint32_t)a.self_ids[q] : 0xffffffffu;
   auto gget = [&](int i) -> uint64_t {
     int lo = 0, hi = nsrc - 1;     // last source whose offset is <= i
     while (lo < hi) {
@@ -675,7 +860,9 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
       if (off[mid] <= i) lo = mid; else hi = mid - 1;
     }
     const int j = i - off[lo];
-    return lo == 0 ? cin[j] : sub[(long)(lo - 1) * a.sub_cap + j];
+    if (lo == 0) return cin[j];
+    const uint64_t e = sub[(long)(lo - 1) * a.sub_cap + j];
+    return (uint32_t)e == self ? ~0ull : pool_key(e);
   };
   uint64_t* cout = a.carry_out + (long)q * a.carry_cap;
   auto compact = [&](auto get, uint64_t E) {
@@ -684,7 +871,7 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
       const int i = base + lane;
       uint64_t e = 0;
       bool keep = false;
-      if (i < N) { e = get(i); keep = e <= E; }
+      if (i < N) { e = get(i); keep = e <= E && e != ~0ull; }
       const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
       const int pos = outpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
@@ -694,8 +881,8 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
     return outpos;
   };
   if (N <= K) {
-    for (int i = lane; i < N; i += 64) cout[i] = gget(i);
-    if (lane == 0) a.carry_out_cnt[q] = N;
+    const int kept = compact(gget, ~0ull);   // everything but a dropped self row
+    if (lane == 0) a.carry_out_cnt[q] = kept;
     return;
   }
   uint64_t E;
@@ -1026,17 +1213,21 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-template <int PREC, int QB, int DC, int SR>
-static hipError_t scan_one(const ScanArgs& a, int groups, hipStream_t stream) {
+static hipError_t scan_grid(void (*kern)(ScanArgs), const ScanArgs& a, int groups, hipStream_t stream) {
   if (groups % 4) return hipErrorInvalidValue;  // a workgroup runs 4 query groups
   if (a.xcd_swizzle) {
     if (a.wg_rows != groups / 4) return hipErrorInvalidValue;
     const long total = (long)a.wg_rows * ((a.slices + 7) / 8 * 8);
-    hipLaunchKernelGGL((scan_kernel<PREC, QB, DC, SR>), dim3((unsigned)total), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(kern, dim3((unsigned)total), dim3(256), 0, stream, a);
   } else {
-    hipLaunchKernelGGL((scan_kernel<PREC, QB, DC, SR>), dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(kern, dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
   }
   return hipGetLastError();
+}
+
+template <int PREC, int QB, int DC, int SR>
+static hipError_t scan_one(const ScanArgs& a, int groups, hipStream_t stream) {
+  return scan_grid(scan_kernel<PREC, QB, DC, SR>, a, groups, stream);
 }
 
 bool scan_supported(int prec, int qb, int dc) {
@@ -1054,6 +1245,15 @@ int scan_stage_rows(int prec, int dp, int dc) {
 hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, hipStream_t stream) {
   const int sr = scan_stage_rows(prec, a.dp, dc);
   if ((a.row_begin % sr) || (a.slice_rows % sr)) return hipErrorInvalidValue;
+  if (a.row_end > 0x7fffff80L) return hipErrorInvalidValue;   // int32 row indices
+  if (prec == PREC_BF16 && a.dp == dc && dc != 96 && !std::getenv("KNNS_SCAN_V2")) {
+#define KNNS_SWZ_CASE(B, D) \
+    if (qb == B && dc == D) return scan_grid(scan_bf16_kernel<B, D>, a, groups, stream);
+    KNNS_SWZ_CASE(1, 32) KNNS_SWZ_CASE(1, 64) KNNS_SWZ_CASE(1, 128)
+    KNNS_SWZ_CASE(2, 32) KNNS_SWZ_CASE(2, 64) KNNS_SWZ_CASE(2, 128)
+#undef KNNS_SWZ_CASE
+    return hipErrorInvalidValue;
+  }
 #define KNNS_SCAN_CASE(P, B, D, S) \
   if (prec == P && qb == B && dc == D && sr == S) return scan_one<P, B, D, S>(a, groups, stream);
 #define KNNS_SCAN_DC(P, B, S) \
