@@ -77,8 +77,9 @@ def main() -> int:
     Id = torch.empty((n, k), dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
 
-    def run(prec: str):
-        corpus.search_device(Qd, n, k, Dd, Id, precision=prec, shortlist=args.shortlist)
+    def run(prec: str, kernel_times: bool = False):
+        corpus.search_device(Qd, n, k, Dd, Id, precision=prec, shortlist=args.shortlist,
+                             kernel_times=kernel_times)
         return knns.last_stats()
 
     if args.copy_calib:
@@ -93,24 +94,30 @@ def main() -> int:
     for _ in range(args.warmup):
         run("bf16")
     torch.cuda.synchronize()
-    lat, scan_s, stats = [], 0.0, None
+    lat, stats = [], None
     t_start = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
         stats = run("bf16")
         lat.append(time.perf_counter() - s0)
-        scan_s += stats["scan_s"]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     qps = n * args.steps / elapsed
     ms_per_step = elapsed / args.steps * 1e3
-    scan_per_step = scan_s / args.steps
+    I_bf16 = Id.cpu().numpy().copy()
+    D_bf16 = Dd.cpu().numpy().copy()
+    # scan-kernel device time, HIP events around every launch on the engine stream (a separate
+    # loop: the per-launch events add a little time to a search, so the q/s loop runs without them)
+    ksteps = max(1, min(args.steps, 5))
+    scan_s = 0.0
+    for _ in range(ksteps):
+        kst = run("bf16", kernel_times=True)
+        scan_s += kst["scan_s"]
+    scan_per_step = scan_s / ksteps
     flops = 2.0 * n * m * d
     achieved = flops / scan_per_step / 1e12
     log(f"bf16: {ms_per_step:.3f} ms/step, {qps:.0f} q/s, scan {scan_per_step * 1e3:.3f} ms "
-        f"({achieved:.1f} TFLOP/s), stats {stats}")
-    I_bf16 = Id.cpu().numpy().copy()
-    D_bf16 = Dd.cpu().numpy().copy()
+        f"({achieved:.1f} TFLOP/s), select {kst['select_s'] * 1e3:.3f} ms, stats {stats}")
 
     # ---- plain FP32 path on the same data ----
     fp32 = None
@@ -120,10 +127,10 @@ def main() -> int:
         f0 = time.perf_counter()
         fscan = 0.0
         for _ in range(args.fp32_steps):
-            st = run("fp32")
-            fscan += st["scan_s"]
+            run("fp32")
         torch.cuda.synchronize()
         fel = time.perf_counter() - f0
+        fscan = run("fp32", kernel_times=True)["scan_s"] * args.fp32_steps
         fp32 = {"qps": n * args.fp32_steps / fel, "ms_per_step": fel / args.fp32_steps * 1e3,
                 "scan_ms": fscan / args.fp32_steps * 1e3,
                 "scan_tflops": flops / (fscan / args.fp32_steps) / 1e12}
@@ -188,7 +195,7 @@ def main() -> int:
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "quality": quality,
-        "engine": stats,
+        "engine": {**stats, "scan_s": kst["scan_s"], "select_s": kst["select_s"]},
     }
     print(json.dumps(out))
     corpus.close()

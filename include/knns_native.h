@@ -49,6 +49,8 @@ typedef enum {
 #define KNNS_DEVICE_PTRS 0x1u   /* queries and outputs are device memory */
 #define KNNS_EXCLUDE_SELF 0x2u  /* query i never returns corpus row self_offset + i */
 #define KNNS_DIST_F64 0x4u      /* write double distances (default: float, or double for FP64) */
+#define KNNS_KERNEL_TIMES 0x8u  /* record per-kernel device times (knns_stats scan_s / select_s); adds
+                                   an event per launch, so leave it off when timing whole searches */
 
 typedef struct knns_corpus_s* knns_corpus;
 
@@ -57,7 +59,7 @@ typedef struct {
   int select;        /* knns_select */
   int k;             /* neighbours per query (ignored by knns_search_multi) */
   int shortlist;     /* candidates kept before the rerank; 0 = precision default */
-  unsigned flags;    /* KNNS_DEVICE_PTRS | KNNS_EXCLUDE_SELF | KNNS_DIST_F64 */
+  unsigned flags;    /* KNNS_DEVICE_PTRS | KNNS_EXCLUDE_SELF | KNNS_DIST_F64 | KNNS_KERNEL_TIMES */
   long self_offset;  /* with KNNS_EXCLUDE_SELF: query i is corpus row self_offset + i */
   int ratio;         /* corpus growth factor between threshold rounds; 0 = planner */
   int slices;        /* corpus slices per query; 0 = planner */
@@ -66,8 +68,8 @@ typedef struct {
 
 typedef struct {
   double total_s;    /* whole call, device time */
-  double scan_s;     /* fused distance + filter kernels */
-  double select_s;   /* merge + final (rerank) kernels */
+  double scan_s;     /* fused distance + filter kernels (with KNNS_KERNEL_TIMES, else 0) */
+  double select_s;   /* merge + final (rerank) kernels (with KNNS_KERNEL_TIMES, else 0) */
   long rounds;
   long slow_compactions;
   long candidates;   /* total entries appended by the scan kernels */

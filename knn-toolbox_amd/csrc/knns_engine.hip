@@ -218,7 +218,9 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
     r.begin = b;
     r.end = e;
     const long rows = e - b;
-    int s = (int)std::min<long>(smax, std::max<long>(1, rows / 256));
+    // small rounds: at least one stage per slice, as many slices as that allows (more waves
+    // share the latency-bound dense appends of the first rounds)
+    int s = (int)std::min<long>(smax, std::max<long>(1, rows / gr));
     r.slice_rows = round_up((rows + s - 1) / s, gr);
     r.slices = (int)((rows + r.slice_rows - 1) / r.slice_rows);
     smax_used = std::max(smax_used, r.slices);
@@ -267,6 +269,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   const bool dev = (p->flags & KNNS_DEVICE_PTRS) != 0;
   const bool excl = (p->flags & KNNS_EXCLUDE_SELF) != 0;
   const bool out64 = c->dtype == KNNS_F64 || (p->flags & KNNS_DIST_F64);
+  const bool ktimes = (p->flags & KNNS_KERNEL_TIMES) != 0;
   hipStream_t st = p->stream ? (hipStream_t)p->stream : E.stream;
   const long m = c->m;
 
@@ -419,10 +422,10 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     sa.wg_rows = P.groups / 4;
     sa.xcd_swizzle = (R.slices >= 8 && std::getenv("KNNS_XCD_SWIZZLE")) ? 1 : 0;
     const size_t e0 = ev++, e1 = ev++;
-    KNNS_HIP(hipEventRecord(E.event(e0), st));
+    if (ktimes) KNNS_HIP(hipEventRecord(E.event(e0), st));
     KNNS_HIP(launch_scan(P.prec_kernel, P.qb, P.dc, sa, P.groups, st));
-    KNNS_HIP(hipEventRecord(E.event(e1), st));
-    scan_ev.push_back({e0, e1});
+    if (ktimes) KNNS_HIP(hipEventRecord(E.event(e1), st));
+    if (ktimes) scan_ev.push_back({e0, e1});
 
     MergeArgs ma{};
     ma.carry_in = (const uint64_t*)carry[cur];
@@ -443,8 +446,8 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     ma.self_ids = (const int*)selfd;
     const size_t e2 = ev++;
     KNNS_HIP(launch_merge(ma, st));
-    KNNS_HIP(hipEventRecord(E.event(e2), st));
-    sel_ev.push_back({e1, e2});
+    if (ktimes) KNNS_HIP(hipEventRecord(E.event(e2), st));
+    if (ktimes) sel_ev.push_back({e1, e2});
     cur ^= 1;
   }
 
@@ -466,11 +469,11 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   fa.out_idx = (int32_t*)oidx;
   fa.n = (int)n;
   const size_t e_pre = ev++;
-  KNNS_HIP(hipEventRecord(E.event(e_pre), st));
+  if (ktimes) KNNS_HIP(hipEventRecord(E.event(e_pre), st));
   const size_t e3 = ev++;
   KNNS_HIP(launch_final(fa, st));
-  KNNS_HIP(hipEventRecord(E.event(e3), st));
-  sel_ev.push_back({e_pre, e3});
+  if (ktimes) KNNS_HIP(hipEventRecord(E.event(e3), st));
+  if (ktimes) sel_ev.push_back({e_pre, e3});
 
   int flags_h[16] = {0};
   KNNS_HIP(hipMemcpyAsync(flags_h, flags, sizeof(flags_h), hipMemcpyDeviceToHost, st));

@@ -93,19 +93,19 @@ __device__ __forceinline__ void group_sync() {
   if constexpr (WAVE) wave_sync(); else __syncthreads();
 }
 
-template <bool WAVE, class Get>
-__device__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
+// each(f) calls f(e) for every entry this thread holds (the entries of the group are the union).
+template <bool WAVE, class Each>
+__device__ uint64_t group_kth_each(Each each, int N, int K, uint32_t* lds) {
   uint32_t* hist = lds;                          // 256 bins
   uint64_t* red = (uint64_t*)(lds + 256);        // 2 x (waves) u64, waves <= 4
   const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   const int nt = WAVE ? 64 : (int)blockDim.x;
   const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   uint64_t mn = ~0ull, mx = 0;
-  for (int i = tid; i < N; i += nt) {
-    uint64_t e = get(i);
+  each([&](uint64_t e) {
     mn = e < mn ? e : mn;
     mx = e > mx ? e : mx;
-  }
+  });
   mn = wave_min_u64(mn);
   mx = wave_max_u64(mx);
   if (lane == 0) { red[wv] = mn; red[4 + wv] = mx; }
@@ -126,13 +126,12 @@ __device__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
   for (int level = 0; level < 9; ++level) {
     for (int b = tid; b < 256; b += nt) hist[b] = 0;
     group_sync<WAVE>();
-    for (int i = tid; i < N; i += nt) {
-      uint64_t e = get(i);
+    each([&](uint64_t e) {
       if (e >= base) {
         uint64_t r = (e - base) >> sh;
         if (r < 256) atomicAdd(&hist[(int)r], 1u);
       }
-    }
+    });
     group_sync<WAVE>();
     // wave 0 finds the bin holding the K-th entry
     if (wv == 0) {
@@ -170,10 +169,9 @@ __device__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
       // every entry of the bin is kept: the K-th entry is the bin's maximum
       const uint64_t hi = base + ((1ull << sh) - 1);
       uint64_t m2 = 0;
-      for (int i = tid; i < N; i += nt) {
-        uint64_t e = get(i);
+      each([&](uint64_t e) {
         if (e >= base && e <= hi) m2 = e > m2 ? e : m2;
-      }
+      });
       m2 = wave_max_u64(m2);
       if (lane == 0) red[4 + wv] = m2;
       group_sync<WAVE>();
@@ -186,6 +184,13 @@ __device__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
     sh = sh > 8 ? sh - 8 : 0;
   }
   return result;
+}
+
+template <bool WAVE, class Get>
+__device__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
+  const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int nt = WAVE ? 64 : (int)blockDim.x;
+  return group_kth_each<WAVE>([&](auto f) { for (int i = tid; i < N; i += nt) f(get(i)); }, N, K, lds);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -928,6 +933,125 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
   }
 }
 
+// Threshold-mode merge with the entries in registers (up to 16 per lane, N <= 1024): the LDS
+// footprint per wave is the source table and the radix histogram only, so several times more
+// waves are resident than with an LDS copy of the entries, and each entry finds its source with
+// one LDS read of a per-entry source map instead of a binary search over the source offsets.
+constexpr int MR_U = 16;
+
+__global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
+  __shared__ int off_all[4][MERGE_MAX_SRC + 1];
+  __shared__ uint16_t map_all[4][64 * MR_U];
+  __shared__ __attribute__((aligned(16))) uint32_t kth_all[4][256 + 32];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = blockIdx.x * 4 + wv;
+  if (q >= a.n) return;   // wave-uniform; only wave-level synchronisation below
+  int* off = off_all[wv];
+  uint16_t* map = map_all[wv];
+  const int nsrc = 1 + 2 * a.slices;
+  const long subq = (long)q * a.slices_max * 2;
+  const int P = (nsrc + 63) / 64;   // sources per lane (<= 5)
+  int c[5] = {0, 0, 0, 0, 0};
+  {
+    int tot = 0;
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int sidx = lane * P + u;
+      if (u < P && sidx < nsrc) c[u] = sidx == 0 ? a.carry_in_cnt[q] : a.sub_cnt[subq + sidx - 1];
+      tot += c[u];
+    }
+    const int excl = (int)wave_incl_scan_u32((uint32_t)tot, lane) - tot;
+    int acc = excl;
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int sidx = lane * P + u;
+      if (u < P && sidx < nsrc) off[sidx] = acc;
+      acc += c[u];
+    }
+    if (lane == 63) off[nsrc] = acc;
+    wave_sync();
+  }
+  const int N = off[nsrc];
+  const int K = a.kc[q];
+  const uint64_t* cin = a.carry_in + (long)q * a.carry_cap;
+  const uint64_t* sub = a.sub + subq * a.sub_cap;
+  const uint32_t self = a.self_ids ? (uint32_t)a.self_ids[q] : 0xffffffffu;
+  auto key_of = [&](int s, int j) -> uint64_t {
+    if (s == 0) return cin[j];
+    const uint64_t e = sub[(long)(s - 1) * a.sub_cap + j];
+    return (uint32_t)e == self ? ~0ull : pool_key(e);
+  };
+  uint64_t* cout = a.carry_out + (long)q * a.carry_cap;
+  uint64_t E = ~0ull;
+  int kept = 0;
+  if (N > 64 * MR_U) {
+    // large union (only with unusual shortlists or ratios): radix select over global memory
+    auto gget = [&](int i) -> uint64_t {
+      int lo = 0, hi = nsrc - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid - 1;
+      }
+      return key_of(lo, i - off[lo]);
+    };
+    E = group_kth<true>(gget, N, K, kth_all[wv]);
+    for (int base = 0; base < N; base += 64) {
+      const int i = base + lane;
+      uint64_t e = 0;
+      bool keep = false;
+      if (i < N) { e = gget(i); keep = e <= E && e != ~0ull; }
+      const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+      const int pos = kept + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+      if (keep && pos < a.carry_cap) cout[pos] = e;
+      kept += __popcll(km);
+    }
+  } else {
+    // per-entry source map, then all loads of the lane in flight at once
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      const int sidx = lane * P + u;
+      if (u < P && sidx < nsrc) {
+        const int o = off[sidx];
+        for (int j = 0; j < c[u]; ++j) map[o + j] = (uint16_t)sidx;
+      }
+    }
+    wave_sync();
+    uint64_t v[MR_U];
+#pragma unroll
+    for (int u = 0; u < MR_U; ++u) {
+      const int i = lane + 64 * u;
+      v[u] = ~0ull;
+      if (i < N) {
+        const int s = map[i];
+        v[u] = key_of(s, i - off[s]);
+      }
+    }
+    auto each = [&](auto f) {
+#pragma unroll
+      for (int u = 0; u < MR_U; ++u)
+        if (lane + 64 * u < N) f(v[u]);
+    };
+    if (N > K) E = group_kth_each<true>(each, N, K, kth_all[wv]);
+#pragma unroll
+    for (int u = 0; u < MR_U; ++u) {
+      if (64 * u >= N) break;   // wave-uniform
+      const bool keep = lane + 64 * u < N && v[u] <= E && v[u] != ~0ull;
+      const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+      const int pos = kept + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+      if (keep && pos < a.carry_cap) cout[pos] = v[u];
+      kept += __popcll(km);
+    }
+  }
+  if (lane == 0) {
+    if (kept > a.carry_cap) kept = a.carry_cap;
+    a.carry_out_cnt[q] = kept;
+    if (N > K) {
+      a.thr[q] = (uint32_t)(E >> 32);
+      if (kept != K) atomicAdd(&a.flags[0], 1);
+    }
+  }
+}
+
 // ------------------------------------------------------------------------------------------------
 // final kernel: one 256-thread block per query
 // ------------------------------------------------------------------------------------------------
@@ -1269,7 +1393,10 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
 hipError_t launch_merge(const MergeArgs& a, hipStream_t stream) {
   if (a.n <= 0) return hipSuccess;
   if (1 + 2 * a.slices > MERGE_MAX_SRC) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(merge_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+  if (a.bitonic || std::getenv("KNNS_MERGE_V1"))
+    hipLaunchKernelGGL(merge_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL(merge_reg_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ KNNS_F32, KNNS_F64 = 0, 1
 PREC = {"fp32": 0, "bf16": 1, "fp8": 2, "fp64": 3}
 SELECT = {"auto": 0, "threshold": 1, "heap": 1, "bitonic": 2}
 KNNS_CORPUS_DEVICE_PTR, KNNS_CORPUS_BF16, KNNS_CORPUS_FP8 = 0x1, 0x2, 0x4
-KNNS_DEVICE_PTRS, KNNS_EXCLUDE_SELF, KNNS_DIST_F64 = 0x1, 0x2, 0x4
+KNNS_DEVICE_PTRS, KNNS_EXCLUDE_SELF, KNNS_DIST_F64, KNNS_KERNEL_TIMES = 0x1, 0x2, 0x4, 0x8
 
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "OVERFLOW"}
 
@@ -224,9 +224,11 @@ class Corpus:
 
     def search_device(self, q_ptr, n: int, k: int, dist_ptr, idx_ptr, precision: str = "fp32",
                       select: str = "auto", shortlist: int = 0, ratio: int = 0, slices: int = 0,
-                      stream: int = 0):
-        """Device pointers (ints or torch tensors); synchronous."""
-        p = self._params(precision, k, select, shortlist, False, 0, ratio, slices, KNNS_DEVICE_PTRS)
+                      stream: int = 0, kernel_times: bool = False):
+        """Device pointers (ints or torch tensors); synchronous. kernel_times: fill scan_s and
+        select_s of last_stats() (one event per launch)."""
+        p = self._params(precision, k, select, shortlist, False, 0, ratio, slices,
+                         KNNS_DEVICE_PTRS | (KNNS_KERNEL_TIMES if kernel_times else 0))
         p.stream = stream or None
         st = native().knns_search(self._h, _ptr(q_ptr) if not isinstance(q_ptr, int) else q_ptr,
                                   KNNS_F64 if self.dtype == np.float64 else KNNS_F32, n, ctypes.byref(p),
