@@ -168,6 +168,7 @@ int default_shortlist(int prec, int k) {
 struct Round {
   long begin, end, slice_rows;
   int slices;
+  int dense;   // every row is a candidate (threshold +inf): unfiltered dense writes
 };
 
 struct Plan {
@@ -209,8 +210,12 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   // else x4 while the prefix is under m/16 (cheap rounds, fewer merges) and x2 after (the large
   // rounds, where a tighter threshold saves filter work)
   const long gr = scan_stage_rows(P.prec_kernel, P.dp, P.dc);  // boundary granularity (rows)
+  P.sub_cap = (int)round_up(std::max<long>(cmax + 96, 256), 64);
+  // The first round has no threshold: every row of it is a candidate. It is written densely
+  // (no filter) and made as large as the merge takes in registers (1024 entries per query), which
+  // replaces the two smallest threshold rounds of a pure x4 schedule.
   long b = 0;
-  long n0 = std::min<long>(m, round_up(std::max<long>(2L * cmax, gr), gr));
+  long n0 = std::min<long>(m, round_up(std::max<long>(std::max<long>(2L * cmax, 1024), gr), gr));
   long e = n0;
   int smax_used = 1;
   while (b < m) {
@@ -219,10 +224,11 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
     r.end = e;
     const long rows = e - b;
     // small rounds: at least one stage per slice, as many slices as that allows (more waves
-    // share the latency-bound dense appends of the first rounds)
+    // share the latency-bound appends of the first rounds)
     int s = (int)std::min<long>(smax, std::max<long>(1, rows / gr));
     r.slice_rows = round_up((rows + s - 1) / s, gr);
     r.slices = (int)((rows + r.slice_rows - 1) / r.slice_rows);
+    r.dense = (b == 0 && r.slice_rows <= P.sub_cap) ? 1 : 0;
     smax_used = std::max(smax_used, r.slices);
     P.rounds.push_back(r);
     b = e;
@@ -230,7 +236,6 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
     e = std::min<long>(m, round_up(e * ratio, gr));
   }
   P.slices_max = smax_used;
-  P.sub_cap = (int)round_up(std::max<long>(cmax + 96, 256), 64);
   return P;
 }
 
@@ -420,6 +425,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     sa.self_ids = (const int*)selfd;
     sa.flags = (int*)flags;
     sa.wg_rows = P.groups / 4;
+    sa.dense = R.dense;
     sa.xcd_swizzle = (R.slices >= 8 && std::getenv("KNNS_XCD_SWIZZLE")) ? 1 : 0;
     const size_t e0 = ev++, e1 = ev++;
     if (ktimes) KNNS_HIP(hipEventRecord(E.event(e0), st));

@@ -49,6 +49,8 @@ struct ScanArgs {
   int* flags;            // [0] = overflow counter, [1] = slow-path compactions
   int xcd_swizzle;       // 1: 1-D grid, slice sl runs on XCD sl % 8 (see scan_kernel)
   int wg_rows;           // query workgroup rows (groups / 4) when xcd_swizzle
+  int dense;             // 1: no filter; every row is written at its slice-relative position of the
+                         // (query, slice, half 0) sub-pool (slice_rows <= sub_cap), fills = rows
 };
 
 struct MergeArgs {

@@ -400,6 +400,36 @@ __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& 
   }
 }
 
+// Dense round (every row a candidate, e.g. the first round, whose threshold is +inf): a lane's
+// group of 4 consecutive rows goes to 4 consecutive entries of the query's (slice, half 0)
+// sub-pool at the rows' slice-relative positions, as two 16-byte stores, with no filter and no
+// counter. Rows past the slice end land inside the pool (a stage never exceeds sub_cap rows) and
+// are not counted.
+template <int QB>
+__device__ __forceinline__ void scan_dense_store(const ScanArgs& a, const ScanState<QB>& st,
+                                                 const f32x16 (&acc)[QB][2], long t0, long c0) {
+  const int h4 = ((threadIdx.x & 63) >> 5) * 4;
+  const uint32_t rb = (uint32_t)t0 + (uint32_t)h4;
+  const long pos0 = t0 - c0 + h4;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int q = st.q0 + qb * 32 + (int)(threadIdx.x & 31);
+    uint64_t* pool = a.sub + (((long)q * a.slices_max + st.sl) * 2) * a.sub_cap + pos0;
+#pragma unroll
+    for (int gi = 0; gi < 8; ++gi) {
+      const int cb = gi >> 2, v0 = (gi & 3) * 4;
+      const uint32_t rel = (uint32_t)(cb * 32 + 8 * (gi & 3));
+      // plain float copies: a bit cast applied to a vector element expression reads element 0
+      const float d0 = acc[qb][cb][v0], d1 = acc[qb][cb][v0 + 1];
+      const float d2 = acc[qb][cb][v0 + 2], d3 = acc[qb][cb][v0 + 3];
+      const u32x4 w0 = {rb + rel, __builtin_bit_cast(uint32_t, d0), rb + rel + 1, __builtin_bit_cast(uint32_t, d1)};
+      const u32x4 w1 = {rb + rel + 2, __builtin_bit_cast(uint32_t, d2), rb + rel + 3, __builtin_bit_cast(uint32_t, d3)};
+      *(u32x4*)(pool + rel) = w0;
+      *(u32x4*)(pool + rel + 2) = w1;
+    }
+  }
+}
+
 // LDS stage: SR corpus rows x one chunk of DC dims + the augmentation. A BF16 row keeps only the
 // half-0 augmentation (8 elements: the 3-term -||r~||^2/2 split); half-1 lanes read the same
 // values and multiply them by the query's zero half-1 augmentation. The row stride is padded to an
@@ -604,7 +634,8 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
         }
       }
       if (c == nch - 1) {
-        if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
+        if (a.dense) scan_dense_store<QB>(a, st, acc, t0, c0);
+        else if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
         else scan_epilogue<QB, false>(a, st, acc, t0, c1);
         bool trig = false;
 #pragma unroll
@@ -617,7 +648,8 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
     const int q = (g * QB + qb) * 32 + col;
-    a.sub_cnt[((long)q * a.slices_max + sl) * 2 + h] = st.cnt[qb];
+    const int fill = a.dense ? (h == 0 && c1 > c0 ? (int)(c1 - c0) : 0) : st.cnt[qb];
+    a.sub_cnt[((long)q * a.slices_max + sl) * 2 + h] = fill;
   }
 }
 
@@ -779,7 +811,8 @@ __global__ __launch_bounds__(256, 2) void scan_bf16_kernel(ScanArgs a) {
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
+      if (a.dense) scan_dense_store<QB>(a, st, acc, t0, c0);
+      else if (t0 + 64 <= c1) scan_epilogue<QB, true>(a, st, acc, t0, c1);
       else scan_epilogue<QB, false>(a, st, acc, t0, c1);
     }
     // room for the next stage: at most 64 appends per lane and query block
@@ -791,7 +824,8 @@ __global__ __launch_bounds__(256, 2) void scan_bf16_kernel(ScanArgs a) {
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
     const int q = (g * QB + qb) * 32 + col;
-    a.sub_cnt[((long)q * a.slices_max + sl) * 2 + h] = st.cnt[qb];
+    const int fill = a.dense ? (h == 0 && c1 > c0 ? (int)(c1 - c0) : 0) : st.cnt[qb];
+    a.sub_cnt[((long)q * a.slices_max + sl) * 2 + h] = fill;
   }
 }
 
