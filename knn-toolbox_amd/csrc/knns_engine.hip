@@ -189,8 +189,9 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   P.cmax = cmax;
   // scan workgroups: 4 query groups each, 2 resident per CU. Choose the slice count so that the
   // workgroups of a large round fill whole waves of resident workgroups (a last wave that is
-  // mostly empty leaves the chip idle): the smallest S with >= 4 waves and >= 95% of the last
-  // wave used, else the S with the best fill.
+  // mostly empty leaves the chip idle): the smallest S with >= 5 waves and >= 95% of the last
+  // wave used, else the S with the best fill (measured on 1M x 128, 10k queries: 5 waves beat 4
+  // by 5%, more waves add merge sources for no gain).
   const long wg_rows = P.groups / 4;
   const long cap = (long)cus * 2;
   int smax = 1;
@@ -200,7 +201,7 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
       const long wgs = wg_rows * S;
       const long waves = (wgs + cap - 1) / cap;
       const double fill = (double)wgs / (double)(waves * cap);
-      if (waves >= 4 && fill >= 0.95) { smax = S; best = 2.0; break; }
+      if (waves >= 5 && fill >= 0.95) { smax = S; best = 2.0; break; }
       const double score = fill * std::min<double>(1.0, (double)wgs / (4.0 * cap));
       if (score > best) { best = score; smax = S; }
     }

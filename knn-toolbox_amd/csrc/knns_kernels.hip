@@ -382,18 +382,22 @@ __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& 
     for (int gi = 0; gi < 8; ++gi) {
       if (__builtin_amdgcn_ballot_w64(gm[gi] > phi) == 0) continue;
       if (gm[gi] > phi) {
-        // lanes holding a survivor in this group of 4 rows: append each row unconditionally at
-        // the current fill and advance the fill only for survivors (room for a whole stage is
-        // guaranteed before the stage starts, so the write stays inside the sub-pool)
+        // lanes holding a survivor in this group of 4 rows append each survivor (usually one):
+        // a loop over the set bits of the group's survivor mask, one store per survivor (room
+        // for a whole stage is guaranteed before the stage starts)
         const int cb = gi >> 2, v0 = (gi & 3) * 4;
+        const float d0 = Dv[cb][v0], d1 = Dv[cb][v0 + 1], d2 = Dv[cb][v0 + 2], d3 = Dv[cb][v0 + 3];
+        uint32_t bits = (d0 > phi ? 1u : 0u) | (d1 > phi ? 2u : 0u) | (d2 > phi ? 4u : 0u) |
+                        (d3 > phi ? 8u : 0u);
+        const uint32_t rg = tb + (uint32_t)(cb * 32 + 8 * (gi & 3));
         int cnt = st.cnt[qb];
-#pragma unroll
-        for (int v = v0; v < v0 + 4; ++v) {
-          const float D = Dv[cb][v];
-          const uint32_t rel = (uint32_t)(cb * 32 + 8 * (v >> 2) + (v & 3));
-          pool[cnt] = ((uint64_t)__builtin_bit_cast(uint32_t, D) << 32) | (tb + rel);
-          cnt += D > phi ? 1 : 0;
-        }
+        do {
+          const uint32_t b = (uint32_t)__builtin_ctz(bits);
+          bits &= bits - 1;
+          const float dl = (b & 1u) ? d1 : d0, dh = (b & 1u) ? d3 : d2;
+          const float D = (b & 2u) ? dh : dl;
+          pool[cnt++] = ((uint64_t)__builtin_bit_cast(uint32_t, D) << 32) | (rg + b);
+        } while (bits);
         st.cnt[qb] = cnt;
       }
     }
