@@ -29,6 +29,18 @@ sys.path.insert(0, str(ROOT / "oracle"))
 
 BF16_PEAK_TFLOPS = 2516.6
 FP32_PEAK_TFLOPS = 157.3
+TRAFFIC_PROFILE = ROOT / "profiles" / "scan_bf16_c2_hbm_traffic.md"
+
+
+def measured_traffic(m: int, n: int, k: int):
+    """Memory traffic of the scan kernel per search (bytes), from the committed FETCH_SIZE +
+    WRITE_SIZE counter runs on this workload (copy-calibrated); None for other shapes."""
+    if (m, n, k) != (1_000_000, 10_000, 100) or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_search:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def log(msg: str) -> None:
@@ -188,7 +200,11 @@ def main() -> int:
                    "m": m, "n": n, "d": d, "k": k,
                    "shortlist": args.shortlist or k + k // 2 + 16, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
+                     "traffic": measured_traffic(m, n, k),
+                     "per": f"one search = the scan kernel's {stats['rounds']} threshold-round "
+                            "launches (2*n*m*d flop over their summed device time; traffic = their "
+                            "copy-calibrated FETCH_SIZE + WRITE_SIZE)"},
         "cpu_baseline": cpu,
         "fp32_path": fp32,
         "speedup_vs_fp32": (qps / fp32["qps"]) if fp32 else None,
