@@ -60,6 +60,7 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-recall", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
+    ap.add_argument("--skip-host-io", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (elementwise read+write) three times; "
                          "its FETCH_SIZE/WRITE_SIZE counters calibrate the HBM-traffic counters")
@@ -151,6 +152,21 @@ def main() -> int:
         fp32["identical_to_bf16"] = bool(np.array_equal(I_fp32, I_bf16))
         log(f"fp32: {fp32}")
 
+    # ---- host-memory boundary: host queries in, host results out (corpus resident) and the whole
+    # drop-in call (gpuknnsHeap: FP32, corpus uploaded and packed inside every call) ----
+    host_io = None
+    if not args.skip_host_io:
+        corpus.search(Q, k, precision="bf16")
+        h0 = time.perf_counter()
+        for _ in range(3):
+            corpus.search(Q, k, precision="bf16")
+        host_io = {"bf16_host_buffers_qps": 3 * n / (time.perf_counter() - h0)}
+        h0 = time.perf_counter()
+        _, _, t_call = knns.gpuknns_heap(Q, R, k)
+        host_io["dropin_heap_call_qps"] = n / (time.perf_counter() - h0)
+        host_io["dropin_heap_call_s"] = t_call
+        log(f"host-memory boundary: {host_io}")
+
     # ---- recall vs the FP64 oracle on a query sample ----
     quality = {}
     if not args.skip_recall:
@@ -208,6 +224,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "fp32_path": fp32,
         "speedup_vs_fp32": (qps / fp32["qps"]) if fp32 else None,
+        "host_boundary": host_io,
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "quality": quality,
