@@ -1,6 +1,6 @@
 """Build recipe for the in-tree HIP libraries (gfx950).
 
-    python knn-toolbox_amd/build.py            # builds lib/libknns_amd.so, libcuknn.so, libcuknn_double.so
+    python knn-toolbox_amd/build.py   # lib/libknns_amd.so, libcuknn.so, libcuknn_double.so, bin/knnTest{,Double}
 
 Outputs go to knn-toolbox_amd/lib/ (git-ignored, shipped with the tree). Every HIP source is built
 with -packed-fp32-ops disabled on the device side; the host pass warns that it does not know the
@@ -18,6 +18,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "lib"
+BIN = PKG / "bin"
 INC = PKG.parent / "include"
 ARCH = os.environ.get("KNNS_OFFLOAD_ARCH", "gfx950")
 
@@ -58,6 +59,14 @@ def build(verbose: bool = False) -> list[Path]:
         _run([HIPCC, *COMMON, *DEVICE_FLAGS, *extra, "-shared", str(CSRC / "cuknns_dropin.cpp"),
               "-L", str(LIB), "-lknns_amd", "-Wl,-rpath,$ORIGIN", "-o", str(so)])
         outs.append(so)
+    # knnTest / knnTestDouble: the file-driven C driver (host code only) over the drop-in libraries
+    BIN.mkdir(exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    for name, lib, extra in [("knnTest", "cuknn", []), ("knnTestDouble", "cuknn_double", ["-D__DOUBLE__"])]:
+        exe = BIN / name
+        _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", str(INC), *extra, str(PKG / "cli" / "knnTest.cpp"),
+              "-L", str(LIB), f"-l{lib}", "-Wl,-rpath,$ORIGIN/../lib", "-o", str(exe)])
+        outs.append(exe)
     for o in objs:
         o.unlink(missing_ok=True)
     if verbose:

@@ -281,3 +281,26 @@ def gpuknns_bitonic(query, data, k: int, double: bool = False, num_streams: int 
 
 def gpuknns_heap(query, data, k: int, double: bool = False, num_streams: int = 1):
     return _dropin_call("gpuknnsHeap", query, data, k, double, num_streams)
+
+
+# ---- raw .bin files of the knnTest driver (src/knnTest.cu:17-48, demo/exportData.m,
+# demo/importData.m) ----
+def knntest_path(double: bool = False) -> Path:
+    """The built knnTest (float knntype) or knnTestDouble executable."""
+    return Path(__file__).resolve().parent / "bin" / ("knnTestDouble" if double else "knnTest")
+
+
+def export_bin(path, rows, double: bool = False) -> None:
+    """Write rows (n x d, row-major = MATLAB's d x n column-major) as raw knntype values."""
+    np.ascontiguousarray(rows, dtype=np.float64 if double else np.float32).tofile(str(path))
+
+
+def import_results(nq: int, k: int, double: bool = False, directory=".") -> tuple:
+    """Read KNNdist.bin / KNNidx.bin written by knnTest: (dist (nq, k), idx (nq, k) as knntype)."""
+    kt = np.float64 if double else np.float32
+    d = Path(directory)
+    dist = np.fromfile(str(d / "KNNdist.bin"), dtype=kt)
+    idx = np.fromfile(str(d / "KNNidx.bin"), dtype=kt)
+    if dist.size != nq * k or idx.size != nq * k:
+        raise KnnsError(KNNS_ERR_INVALID, f"result files hold {dist.size}/{idx.size} values, expected {nq * k}")
+    return dist.reshape(nq, k), idx.reshape(nq, k)
