@@ -15,24 +15,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/knns_native.h"
-#include "knns_internal.h"
-
-struct knns_corpus_s {
-  long m = 0;
-  int d = 0;
-  int dp = 0, dc = 0;
-  long rows_pad = 0;
-  long ld32 = 0, ld16 = 0;
-  int dtype = KNNS_F32;
-  unsigned flags = 0;
-  float* x32 = nullptr;
-  uint16_t* x16 = nullptr;
-  double* x64 = nullptr;
-};
+#include "knns_engine.h"
 
 namespace knns {
-namespace {
 
 thread_local std::string g_err;
 
@@ -45,6 +30,8 @@ knns_status fail(knns_status st, const char* fmt, ...) {
   g_err = buf;
   return st;
 }
+
+namespace {
 
 #define KNNS_HIP(expr)                                                                      \
   do {                                                                                      \
@@ -145,7 +132,7 @@ Engine& engine() {
   return *e;
 }
 
-inline long round_up(long x, long a) { return (x + a - 1) / a * a; }
+}  // namespace
 
 // padded dimension and MFMA chunk for a dimension d
 void dims_for(int d, int* dp, int* dc) {
@@ -164,17 +151,6 @@ int default_shortlist(int prec, int k) {
     default: return k + 16;
   }
 }
-
-struct Round {
-  long begin, end, slice_rows;
-  int slices;
-  int dense;   // every row is a candidate (threshold +inf): unfiltered dense writes
-};
-
-struct Plan {
-  int prec_kernel, qb, dc, dp, groups, n_pad, slices_max, sub_cap, kmax, cmax;
-  std::vector<Round> rounds;
-};
 
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
                int slices_req) {
@@ -239,6 +215,139 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   P.slices_max = smax_used;
   return P;
 }
+
+SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long total_out, bool excl,
+                         bool out64) {
+  const long n_pad = P.n_pad;
+  SearchSizes z{};
+  z.q32 = (size_t)n_pad * c->ld32 * 4;
+  z.q16 = P.prec_kernel == PREC_BF16 ? (size_t)n_pad * c->ld16 * 2 : 0;
+  z.q64 = c->dtype == KNNS_F64 ? (size_t)n * c->d * 8 : 0;
+  z.thr = (size_t)n_pad * 4;
+  z.kc = (size_t)n_pad * 4;
+  z.kq = (size_t)n * 4;
+  z.off = (size_t)n * 8;
+  z.self = excl ? (size_t)n_pad * 4 : 0;
+  z.sub = (size_t)n_pad * P.slices_max * 2 * P.sub_cap * 8;
+  z.subcnt = (size_t)n_pad * P.slices_max * 2 * 4;
+  z.carry = (size_t)n * P.cmax * 8;
+  z.ccnt = (size_t)n * 4;
+  z.flags = 16 * 4;
+  z.odist = (size_t)total_out * (out64 ? 8 : 4);
+  z.oidx = (size_t)total_out * 4;
+  return z;
+}
+
+hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStream_t st,
+                          const LaunchHook* hook) {
+  const knns_corpus_s* c = L.c;
+  const Plan& P = *L.P;
+  const long n = L.n, n_pad = P.n_pad, m = c->m;
+  hipError_t e;
+#define KNNS_TRY(x) \
+  if ((e = (x)) != hipSuccess) return e
+  // queries -> packed operand rows
+  PackArgs pa{};
+  pa.src = L.qsrc;
+  pa.src_f64 = L.dtype == KNNS_F64;
+  pa.rows = n;
+  pa.rows_pad = n_pad;
+  pa.d = c->d;
+  pa.ld_src = c->d;
+  pa.x32 = (float*)B.q32;
+  pa.ld32 = c->ld32;
+  pa.x16 = (uint16_t*)B.q16;
+  pa.ld16 = c->ld16;
+  pa.x64 = (double*)B.q64;
+  pa.dp = c->dp;
+  pa.is_query = 1;
+  KNNS_TRY(launch_pack(pa, st));
+  KNNS_TRY(launch_init_queries((uint32_t*)B.thr, (int*)B.kc, (int*)B.kq, (long*)B.off,
+                               (int*)B.self, (int*)B.ccnt[0], n, n_pad, L.uniform ? L.kuni : 1,
+                               L.cmax, L.self_offset, m, L.n_dev, L.uniform ? 1 : 0, st));
+  KNNS_TRY(launch_fill_i32((int*)B.flags, 0, 16, st));
+
+  const void* xop = P.prec_kernel == PREC_BF16 ? (const void*)c->x16 : (const void*)c->x32;
+  const void* qop = P.prec_kernel == PREC_BF16 ? B.q16 : B.q32;
+  const long ldx = P.prec_kernel == PREC_BF16 ? c->ld16 : c->ld32;
+  int cur = 0;
+  for (size_t r = 0; r < P.rounds.size(); ++r) {
+    const Round& R = P.rounds[r];
+    ScanArgs sa{};
+    sa.X = xop;
+    sa.Qp = qop;
+    sa.ldx = ldx;
+    sa.ldq = ldx;
+    sa.dp = P.dp;
+    sa.row_begin = R.begin;
+    sa.row_end = R.end;
+    sa.slice_rows = R.slice_rows;
+    sa.slices = R.slices;
+    sa.slices_max = P.slices_max;
+    sa.n_pad = (int)n_pad;
+    sa.thr = (const uint32_t*)B.thr;
+    sa.kc = (const int*)B.kc;
+    sa.sub = (uint64_t*)B.sub;
+    sa.sub_cnt = (int*)B.subcnt;
+    sa.sub_cap = P.sub_cap;
+    sa.self_ids = (const int*)B.self;
+    sa.flags = (int*)B.flags;
+    sa.wg_rows = P.groups / 4;
+    sa.dense = R.dense;
+    sa.xcd_swizzle = (R.slices >= 8 && std::getenv("KNNS_XCD_SWIZZLE")) ? 1 : 0;
+    if (hook) (*hook)(0);
+    KNNS_TRY(launch_scan(P.prec_kernel, P.qb, P.dc, sa, P.groups, st));
+    if (hook) (*hook)(1);
+
+    MergeArgs ma{};
+    ma.carry_in = (const uint64_t*)B.carry[cur];
+    ma.carry_in_cnt = (const int*)B.ccnt[cur];
+    ma.carry_out = (uint64_t*)B.carry[cur ^ 1];
+    ma.carry_out_cnt = (int*)B.ccnt[cur ^ 1];
+    ma.carry_cap = P.cmax;
+    ma.sub = (const uint64_t*)B.sub;
+    ma.sub_cnt = (const int*)B.subcnt;
+    ma.sub_cap = P.sub_cap;
+    ma.slices = R.slices;
+    ma.slices_max = P.slices_max;
+    ma.thr = (uint32_t*)B.thr;
+    ma.kc = (const int*)B.kc;
+    ma.n = (int)n;
+    ma.bitonic = L.bitonic;
+    ma.flags = (int*)B.flags;
+    ma.self_ids = (const int*)B.self;
+    ma.n_dev = L.n_dev;
+    KNNS_TRY(launch_merge(ma, st));
+    if (hook) (*hook)(2);
+    cur ^= 1;
+  }
+
+  FinalArgs fa{};
+  fa.carry = (const uint64_t*)B.carry[cur];
+  fa.carry_cnt = (const int*)B.ccnt[cur];
+  fa.carry_cap = P.cmax;
+  fa.kc = (const int*)B.kc;
+  fa.kq = (const int*)B.kq;
+  fa.out_off = (const long*)B.off;
+  if (c->dtype == KNNS_F64) {
+    fa.Xm = c->x64; fa.ldxm = c->d; fa.Qm = B.q64; fa.ldqm = c->d; fa.master_f64 = 1;
+  } else {
+    fa.Xm = c->x32; fa.ldxm = c->ld32; fa.Qm = B.q32; fa.ldqm = c->ld32; fa.master_f64 = 0;
+  }
+  fa.d = c->d;
+  fa.out_f64 = L.out64 ? 1 : 0;
+  fa.out_dist = B.odist;
+  fa.out_idx = (int32_t*)B.oidx;
+  fa.n = (int)n;
+  fa.n_dev = L.n_dev;
+  if (hook) (*hook)(3);
+  KNNS_TRY(launch_final(fa, st));
+  if (hook) (*hook)(4);
+#undef KNNS_TRY
+  return hipSuccess;
+}
+
+namespace {
 
 bool is_device_ptr(const void* p) {
   hipPointerAttribute_t at;
@@ -314,72 +423,51 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
 
   Plan P = make_plan(m, c->d, n, kmax, cmax, prec, E.cus, p->ratio, p->slices);
   if (P.dp != c->dp) return fail(KNNS_ERR_INVALID, "corpus/plan dimension mismatch");
-  const long n_pad = P.n_pad;
 
-  // workspace
-  void *q32, *q16 = nullptr, *q64 = nullptr, *thr, *kcd, *kqd, *offd, *selfd = nullptr, *sub,
-             *subcnt, *car0, *car1, *cc0, *cc1, *flags, *odist, *oidx;
+  // workspace (grow-only engine slots)
+  const SearchSizes z = search_sizes(c, P, n, total_out, excl, out64);
+  SearchBuffers B;
   knns_status sst;
 #define KNNS_WS(slot, bytes, ptr) \
   if ((sst = E.get(slot, (size_t)(bytes), &ptr)) != KNNS_OK) return sst
-  KNNS_WS(W_Q32, n_pad * c->ld32 * 4, q32);
-  if (P.prec_kernel == PREC_BF16) KNNS_WS(W_Q16, n_pad * c->ld16 * 2, q16);
-  if (c->dtype == KNNS_F64) KNNS_WS(W_Q64, n * (long)c->d * 8, q64);
-  KNNS_WS(W_THR, n_pad * 4, thr);
-  KNNS_WS(W_KC, n_pad * 4, kcd);
-  KNNS_WS(W_KQ, n * 4, kqd);
-  KNNS_WS(W_OFF, n * 8, offd);
-  if (excl) KNNS_WS(W_SELF, n_pad * 4, selfd);
-  KNNS_WS(W_SUB, (size_t)n_pad * P.slices_max * 2 * P.sub_cap * 8, sub);
-  KNNS_WS(W_SUBCNT, (size_t)n_pad * P.slices_max * 2 * 4, subcnt);
-  KNNS_WS(W_CARRY0, (size_t)n * P.cmax * 8, car0);
-  KNNS_WS(W_CARRY1, (size_t)n * P.cmax * 8, car1);
-  KNNS_WS(W_CCNT0, n * 4, cc0);
-  KNNS_WS(W_CCNT1, n * 4, cc1);
-  KNNS_WS(W_FLAGS, 16 * 4, flags);
+  KNNS_WS(W_Q32, z.q32, B.q32);
+  if (z.q16) KNNS_WS(W_Q16, z.q16, B.q16);
+  if (z.q64) KNNS_WS(W_Q64, z.q64, B.q64);
+  KNNS_WS(W_THR, z.thr, B.thr);
+  KNNS_WS(W_KC, z.kc, B.kc);
+  KNNS_WS(W_KQ, z.kq, B.kq);
+  KNNS_WS(W_OFF, z.off, B.off);
+  if (z.self) KNNS_WS(W_SELF, z.self, B.self);
+  KNNS_WS(W_SUB, z.sub, B.sub);
+  KNNS_WS(W_SUBCNT, z.subcnt, B.subcnt);
+  KNNS_WS(W_CARRY0, z.carry, B.carry[0]);
+  KNNS_WS(W_CARRY1, z.carry, B.carry[1]);
+  KNNS_WS(W_CCNT0, z.ccnt, B.ccnt[0]);
+  KNNS_WS(W_CCNT1, z.ccnt, B.ccnt[1]);
+  KNNS_WS(W_FLAGS, z.flags, B.flags);
   if (!dev) {
-    KNNS_WS(W_ODIST, total_out * (out64 ? 8 : 4), odist);
-    KNNS_WS(W_OIDX, total_out * 4, oidx);
+    KNNS_WS(W_ODIST, z.odist, B.odist);
+    KNNS_WS(W_OIDX, z.oidx, B.oidx);
   } else {
-    odist = dist;
-    oidx = idx;
+    B.odist = dist;
+    B.oidx = idx;
   }
-#undef KNNS_WS
 
   size_t ev = 0;
   hipEvent_t e_begin = E.event(ev++);
   if (!e_begin) return fail(KNNS_ERR_DEVICE, "event creation failed");
   KNNS_HIP(hipEventRecord(e_begin, st));
 
-  // queries -> packed operand rows
   const size_t qsz = dtype == KNNS_F64 ? 8 : 4;
   const void* qsrc = queries;
   if (!dev) {
     void* stage;
-    if ((sst = E.get(W_STAGE, n * (long)c->d * qsz, &stage)) != KNNS_OK) return sst;
+    KNNS_WS(W_STAGE, n * (long)c->d * qsz, stage);
     KNNS_HIP(hipMemcpyAsync(stage, queries, n * (long)c->d * qsz, hipMemcpyHostToDevice, st));
     qsrc = stage;
   }
-  PackArgs pa{};
-  pa.src = qsrc;
-  pa.src_f64 = dtype == KNNS_F64;
-  pa.rows = n;
-  pa.rows_pad = n_pad;
-  pa.d = c->d;
-  pa.ld_src = c->d;
-  pa.x32 = (float*)q32;
-  pa.ld32 = c->ld32;
-  pa.x16 = (uint16_t*)q16;
-  pa.ld16 = c->ld16;
-  pa.x64 = (double*)q64;
-  pa.dp = c->dp;
-  pa.is_query = 1;
-  KNNS_HIP(launch_pack(pa, st));
-
-  // per-query control arrays: built on the device (uniform k), per-query values staged through
-  // pinned host memory otherwise; either way no synchronisation inside the search
-  KNNS_HIP(launch_init_queries((uint32_t*)thr, (int*)kcd, (int*)kqd, (long*)offd, (int*)selfd,
-                               (int*)cc0, n, n_pad, uniform ? kuni : 1, cmax, p->self_offset, m, st));
+#undef KNNS_WS
+  // per-query k: staged through pinned host memory (no synchronisation inside the search)
   if (!uniform) {
     void* pin;
     if ((sst = E.pinned(n * 16, &pin)) != KNNS_OK) return sst;
@@ -389,104 +477,47 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     std::memcpy(pk, kq.data(), n * 4);
     std::memcpy(pc, kc.data(), n * 4);
     std::memcpy(po, off.data(), n * 8);
-    KNNS_HIP(hipMemcpyAsync(kqd, pk, n * 4, hipMemcpyHostToDevice, st));
-    KNNS_HIP(hipMemcpyAsync(kcd, pc, n * 4, hipMemcpyHostToDevice, st));
-    KNNS_HIP(hipMemcpyAsync(offd, po, n * 8, hipMemcpyHostToDevice, st));
+    KNNS_HIP(hipMemcpyAsync(B.kq, pk, n * 4, hipMemcpyHostToDevice, st));
+    KNNS_HIP(hipMemcpyAsync(B.kc, pc, n * 4, hipMemcpyHostToDevice, st));
+    KNNS_HIP(hipMemcpyAsync(B.off, po, n * 8, hipMemcpyHostToDevice, st));
   }
-  KNNS_HIP(launch_fill_i32((int*)flags, 0, 16, st));
 
-  const void* xop = P.prec_kernel == PREC_BF16 ? (const void*)c->x16 : (const void*)c->x32;
-  const void* qop = P.prec_kernel == PREC_BF16 ? q16 : q32;
-  const long ldx = P.prec_kernel == PREC_BF16 ? c->ld16 : c->ld32;
-  const int bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
-
+  SearchLaunch L;
+  L.c = c;
+  L.P = &P;
+  L.n = n;
+  L.dtype = dtype;
+  L.qsrc = qsrc;
+  L.uniform = uniform;
+  L.kuni = uniform ? kuni : 1;
+  L.cmax = cmax;
+  L.excl = excl;
+  L.self_offset = p->self_offset;
+  L.out64 = out64;
+  L.bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
+  // per-launch events (KNNS_KERNEL_TIMES): pairs (scan begin, scan end), (scan end, merge end),
+  // (final begin, final end)
   std::vector<std::pair<size_t, size_t>> scan_ev, sel_ev;
-  void* carry[2] = {car0, car1};
-  void* ccnt[2] = {cc0, cc1};
-  int cur = 0;
-  for (size_t r = 0; r < P.rounds.size(); ++r) {
-    const Round& R = P.rounds[r];
-    ScanArgs sa{};
-    sa.X = xop;
-    sa.Qp = qop;
-    sa.ldx = ldx;
-    sa.ldq = ldx;
-    sa.dp = P.dp;
-    sa.row_begin = R.begin;
-    sa.row_end = R.end;
-    sa.slice_rows = R.slice_rows;
-    sa.slices = R.slices;
-    sa.slices_max = P.slices_max;
-    sa.n_pad = (int)n_pad;
-    sa.thr = (const uint32_t*)thr;
-    sa.kc = (const int*)kcd;
-    sa.sub = (uint64_t*)sub;
-    sa.sub_cnt = (int*)subcnt;
-    sa.sub_cap = P.sub_cap;
-    sa.self_ids = (const int*)selfd;
-    sa.flags = (int*)flags;
-    sa.wg_rows = P.groups / 4;
-    sa.dense = R.dense;
-    sa.xcd_swizzle = (R.slices >= 8 && std::getenv("KNNS_XCD_SWIZZLE")) ? 1 : 0;
-    const size_t e0 = ev++, e1 = ev++;
-    if (ktimes) KNNS_HIP(hipEventRecord(E.event(e0), st));
-    KNNS_HIP(launch_scan(P.prec_kernel, P.qb, P.dc, sa, P.groups, st));
-    if (ktimes) KNNS_HIP(hipEventRecord(E.event(e1), st));
-    if (ktimes) scan_ev.push_back({e0, e1});
-
-    MergeArgs ma{};
-    ma.carry_in = (const uint64_t*)carry[cur];
-    ma.carry_in_cnt = (const int*)ccnt[cur];
-    ma.carry_out = (uint64_t*)carry[cur ^ 1];
-    ma.carry_out_cnt = (int*)ccnt[cur ^ 1];
-    ma.carry_cap = P.cmax;
-    ma.sub = (const uint64_t*)sub;
-    ma.sub_cnt = (const int*)subcnt;
-    ma.sub_cap = P.sub_cap;
-    ma.slices = R.slices;
-    ma.slices_max = P.slices_max;
-    ma.thr = (uint32_t*)thr;
-    ma.kc = (const int*)kcd;
-    ma.n = (int)n;
-    ma.bitonic = bitonic;
-    ma.flags = (int*)flags;
-    ma.self_ids = (const int*)selfd;
-    const size_t e2 = ev++;
-    KNNS_HIP(launch_merge(ma, st));
-    if (ktimes) KNNS_HIP(hipEventRecord(E.event(e2), st));
-    if (ktimes) sel_ev.push_back({e1, e2});
-    cur ^= 1;
-  }
-
-  FinalArgs fa{};
-  fa.carry = (const uint64_t*)carry[cur];
-  fa.carry_cnt = (const int*)ccnt[cur];
-  fa.carry_cap = P.cmax;
-  fa.kc = (const int*)kcd;
-  fa.kq = (const int*)kqd;
-  fa.out_off = (const long*)offd;
-  if (c->dtype == KNNS_F64) {
-    fa.Xm = c->x64; fa.ldxm = c->d; fa.Qm = q64; fa.ldqm = c->d; fa.master_f64 = 1;
-  } else {
-    fa.Xm = c->x32; fa.ldxm = c->ld32; fa.Qm = q32; fa.ldqm = c->ld32; fa.master_f64 = 0;
-  }
-  fa.d = c->d;
-  fa.out_f64 = out64 ? 1 : 0;
-  fa.out_dist = odist;
-  fa.out_idx = (int32_t*)oidx;
-  fa.n = (int)n;
-  const size_t e_pre = ev++;
-  if (ktimes) KNNS_HIP(hipEventRecord(E.event(e_pre), st));
-  const size_t e3 = ev++;
-  KNNS_HIP(launch_final(fa, st));
-  if (ktimes) KNNS_HIP(hipEventRecord(E.event(e3), st));
-  if (ktimes) sel_ev.push_back({e_pre, e3});
+  size_t last_scan_end = 0, final_begin = 0;
+  bool ev_ok = true;
+  LaunchHook hook = [&](int what) {
+    const size_t i = ev++;
+    hipEvent_t e = E.event(i);
+    if (!e || hipEventRecord(e, st) != hipSuccess) { ev_ok = false; return; }
+    if (what == 0) scan_ev.push_back({i, 0});
+    else if (what == 1) { scan_ev.back().second = i; last_scan_end = i; }
+    else if (what == 2) sel_ev.push_back({last_scan_end, i});
+    else if (what == 3) final_begin = i;
+    else sel_ev.push_back({final_begin, i});
+  };
+  KNNS_HIP(enqueue_search(L, B, st, ktimes ? &hook : nullptr));
+  if (!ev_ok) return fail(KNNS_ERR_DEVICE, "kernel timing event failed");
 
   int flags_h[16] = {0};
-  KNNS_HIP(hipMemcpyAsync(flags_h, flags, sizeof(flags_h), hipMemcpyDeviceToHost, st));
+  KNNS_HIP(hipMemcpyAsync(flags_h, B.flags, sizeof(flags_h), hipMemcpyDeviceToHost, st));
   if (!dev) {
-    KNNS_HIP(hipMemcpyAsync(dist, odist, total_out * (out64 ? 8 : 4), hipMemcpyDeviceToHost, st));
-    KNNS_HIP(hipMemcpyAsync(idx, oidx, total_out * 4, hipMemcpyDeviceToHost, st));
+    KNNS_HIP(hipMemcpyAsync(dist, B.odist, z.odist, hipMemcpyDeviceToHost, st));
+    KNNS_HIP(hipMemcpyAsync(idx, B.oidx, z.oidx, hipMemcpyDeviceToHost, st));
   }
   const size_t e_end = ev++;
   KNNS_HIP(hipEventRecord(E.event(e_end), st));

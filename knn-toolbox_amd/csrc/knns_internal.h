@@ -70,6 +70,7 @@ struct MergeArgs {
   int bitonic;           // 1: exact truncated sort (keeps exactly K); 0: histogram bound
   int* flags;
   const int* self_ids;   // optional: corpus row excluded for each query (all-kNN), -1 = none
+  const int* n_dev;      // optional: real query count on the device (overrides n)
 };
 
 struct FinalArgs {
@@ -89,6 +90,7 @@ struct FinalArgs {
   void* out_dist;
   int32_t* out_idx;
   int n;
+  const int* n_dev;      // optional: real query count on the device (overrides n)
 };
 
 struct PackArgs {
@@ -115,7 +117,8 @@ hipError_t launch_final(const FinalArgs& a, hipStream_t stream);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t stream);
 hipError_t launch_init_queries(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
                                int* carry_cnt, long n, long n_pad, int k, int shortlist,
-                               long self_offset, long m, hipStream_t stream);
+                               long self_offset, long m, const int* n_dev, int write_k,
+                               hipStream_t stream);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t stream);
 bool scan_supported(int prec, int qb, int dc);
 int scan_stage_rows(int prec, int dp, int dc);

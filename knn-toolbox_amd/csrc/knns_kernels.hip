@@ -862,7 +862,7 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
   __shared__ __attribute__((aligned(16))) uint32_t kth_all[4][256 + 32];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = blockIdx.x * 4 + wv;
-  if (q >= a.n) return;   // wave-uniform; only wave-level synchronisation below
+  if (q >= (a.n_dev ? *a.n_dev : a.n)) return;   // wave-uniform; only wave-level sync below
   uint64_t* ent = ent_all[wv];
   int* off = off_all[wv];
   const int nsrc = 1 + 2 * a.slices;
@@ -983,7 +983,7 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
   __shared__ __attribute__((aligned(16))) uint32_t kth_all[4][256 + 32];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = blockIdx.x * 4 + wv;
-  if (q >= a.n) return;   // wave-uniform; only wave-level synchronisation below
+  if (q >= (a.n_dev ? *a.n_dev : a.n)) return;   // wave-uniform; only wave-level sync below
   int* off = off_all[wv];
   uint16_t* map = map_all[wv];
   const int nsrc = 1 + 2 * a.slices;
@@ -1118,7 +1118,7 @@ __global__ __launch_bounds__(256) void final_block_kernel(FinalArgs a) {
   __shared__ uint32_t ev[FINAL_MAX];
   __shared__ double qrow[4096];
   const int q = blockIdx.x;
-  if (q >= a.n) return;
+  if (q >= (a.n_dev ? *a.n_dev : a.n)) return;
   const int tid = threadIdx.x, nt = blockDim.x;
   int cnt = a.carry_cnt[q];
   if (cnt > FINAL_MAX) cnt = FINAL_MAX;
@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
   __shared__ uint32_t di_all[4][FW_MAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = blockIdx.x * 4 + wv;
-  if (q >= a.n) return;   // wave-uniform
+  if (q >= (a.n_dev ? *a.n_dev : a.n)) return;   // wave-uniform
   uint64_t* dk = dk_all[wv];
   uint32_t* di = di_all[wv];
   int cnt = a.carry_cnt[q];
@@ -1326,30 +1326,38 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
 // per-query control arrays for a uniform-k search (no host upload): thresholds (+inf for real
 // queries, -inf for padding queries so nothing survives), shortlist, k, output offset, self id
 // ------------------------------------------------------------------------------------------------
+// Per-query control state. n_dev (optional) holds the real query count on the device, overriding
+// n: a search captured in a graph for n_pad query slots then serves any count up to n_pad.
+// write_k = 0 leaves kc / kq / off to the caller (per-query k copied from the host).
 __global__ void init_queries_kernel(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
                                     int* carry_cnt, long n, long n_pad, int k, int shortlist,
-                                    long self_offset, long m) {
+                                    long self_offset, long m, const int* n_dev, int write_k) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_pad) return;
-  thr[i] = i < n ? ORD_POS_INF : ORD_NEG_INF;
-  kc[i] = shortlist;
+  const long nr = n_dev ? (long)*n_dev : n;
+  thr[i] = i < nr ? ORD_POS_INF : ORD_NEG_INF;
+  if (write_k) kc[i] = shortlist;
   if (self_ids) {
     const long r = self_offset + i;
-    self_ids[i] = (i < n && r >= 0 && r < m) ? (int)r : -1;
+    self_ids[i] = (i < nr && r >= 0 && r < m) ? (int)r : -1;
   }
-  if (i < n) {
-    kq[i] = k;
-    off[i] = i * (long)k;
+  if (i < nr) {
+    if (write_k) {
+      kq[i] = k;
+      off[i] = i * (long)k;
+    }
     carry_cnt[i] = 0;
   }
 }
 
 hipError_t launch_init_queries(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
                                int* carry_cnt, long n, long n_pad, int k, int shortlist,
-                               long self_offset, long m, hipStream_t stream) {
+                               long self_offset, long m, const int* n_dev, int write_k,
+                               hipStream_t stream) {
   if (n_pad <= 0) return hipSuccess;
   hipLaunchKernelGGL(init_queries_kernel, dim3((unsigned)((n_pad + 255) / 256)), dim3(256), 0, stream,
-                     thr, kc, kq, off, self_ids, carry_cnt, n, n_pad, k, shortlist, self_offset, m);
+                     thr, kc, kq, off, self_ids, carry_cnt, n, n_pad, k, shortlist, self_offset, m,
+                     n_dev, write_k);
   return hipGetLastError();
 }
 
