@@ -103,6 +103,45 @@ int knns_device_count(void);
 /* releases the cached workspace of the engine */
 void knns_release_workspace(void);
 
+/* ---- serving: continuous batching over a resident corpus ----
+ * No counterpart in the reference (its API is one synchronous call per query batch); this is
+ * the serving form of the same search for callers that submit many independent requests.
+ * A server owns one device workspace and one captured HIP graph per batch-capacity bucket
+ * (64, 256, 1024, ... up to max_batch query slots); a worker thread packs every pending request
+ * (FIFO) into the smallest bucket that holds them and replays that bucket's graph. A request's
+ * result does not depend on what it is batched with: it equals knns_search on its own. */
+typedef struct knns_server_s* knns_server;
+
+typedef struct {
+  int precision;       /* knns_precision (FP32, BF16; FP64 for double corpora) */
+  int max_k;           /* largest k a request may ask for, 1..1024 */
+  long max_batch;      /* query slots of the largest bucket (>= 1) */
+  long mem_budget;     /* device bytes the server may hold; 0 = no limit. Buckets whose workspace
+                          does not fit are dropped at creation (the largest first) */
+} knns_server_config;
+
+typedef struct {
+  long requests, queries, batches;
+  double p50_ms, p99_ms, max_ms;  /* request latency: submit to results written */
+  double mean_batch;              /* queries per batch */
+  double busy_s;                  /* time the worker spent running batches */
+  long buckets, max_batch;        /* after the memory budget */
+  long workspace_bytes;
+} knns_server_stats;
+
+knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_server_config* cfg);
+/* Copies the n queries (host memory, dtype of the corpus); results are written to dist (n*k,
+ * float, or double for double corpora) and idx (n*k int32), which must stay valid until
+ * knns_server_wait returns for the ticket. n must not exceed the server's max_batch. */
+knns_status knns_server_submit(knns_server s, const void* queries, int dtype, long n, int k,
+                               void* dist, int32_t* idx, unsigned long long* ticket);
+/* Blocks until the request is done (timeout_s < 0: no limit) and returns its status; a ticket
+ * can be waited for once. KNNS_ERR_INVALID for an unknown ticket, KNNS_ERR_DEVICE on timeout. */
+knns_status knns_server_wait(knns_server s, unsigned long long ticket, double timeout_s);
+knns_status knns_server_stats_get(knns_server s, knns_server_stats* out, int reset);
+/* Finishes the requests already submitted, then frees the server. */
+knns_status knns_server_destroy(knns_server s);
+
 #ifdef __cplusplus
 }
 #endif

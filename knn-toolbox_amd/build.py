@@ -42,7 +42,7 @@ def build(verbose: bool = False) -> list[Path]:
     LIB.mkdir(exist_ok=True)
     objs = []
     jobs = []
-    for src in ["knns_kernels.hip", "knns_engine.hip"]:
+    for src in ["knns_kernels.hip", "knns_engine.hip", "knns_server.hip"]:
         obj = LIB / (Path(src).stem + ".o")
         # Kernels: NaN inputs are outside the supported domain, and without NaN semantics the
         # filter's max trees compile to bare v_max/v_max3 (no per-operand canonicalisation).

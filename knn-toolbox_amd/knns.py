@@ -58,6 +58,21 @@ class Times(ctypes.Structure):
     _fields_ = [("dst_time", ctypes.c_double), ("srch_time", ctypes.c_double), ("knn_time", ctypes.c_double)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("max_k", ctypes.c_int), ("max_batch", ctypes.c_long),
+                ("mem_budget", ctypes.c_long)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_long), ("queries", ctypes.c_long), ("batches", ctypes.c_long),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("mean_batch", ctypes.c_double), ("busy_s", ctypes.c_double), ("buckets", ctypes.c_long),
+                ("max_batch", ctypes.c_long), ("workspace_bytes", ctypes.c_long)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _native = None
 _dropin = {}
 
@@ -99,6 +114,17 @@ def native():
         L.knns_device_count.restype = I
         L.knns_release_workspace.argtypes = []
         L.knns_release_workspace.restype = None
+        SC, SS = ctypes.POINTER(ServerConfig), ctypes.POINTER(ServerStats)
+        L.knns_server_create.argtypes = [ctypes.POINTER(V), V, SC]
+        L.knns_server_create.restype = I
+        L.knns_server_submit.argtypes = [V, V, I, L_, I, V, V, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.knns_server_submit.restype = I
+        L.knns_server_wait.argtypes = [V, ctypes.c_ulonglong, ctypes.c_double]
+        L.knns_server_wait.restype = I
+        L.knns_server_stats_get.argtypes = [V, SS, I]
+        L.knns_server_stats_get.restype = I
+        L.knns_server_destroy.argtypes = [V]
+        L.knns_server_destroy.restype = I
         _native = L
     return _native
 
@@ -304,3 +330,64 @@ def import_results(nq: int, k: int, double: bool = False, directory=".") -> tupl
     if dist.size != nq * k or idx.size != nq * k:
         raise KnnsError(KNNS_ERR_INVALID, f"result files hold {dist.size}/{idx.size} values, expected {nq * k}")
     return dist.reshape(nq, k), idx.reshape(nq, k)
+
+
+class Server:
+    """Continuous-batching server over a resident corpus (knns_server_*): submit() returns a
+    ticket at once, result() blocks until that request's (dist, idx) are ready."""
+
+    def __init__(self, corpus: "Corpus", precision: str = "bf16", max_k: int = 100, max_batch: int = 4096,
+                 mem_budget: int = 0):
+        self._corpus = corpus           # keeps the corpus alive
+        self._dtype = corpus.dtype
+        cfg = ServerConfig(PREC[precision], int(max_k), int(max_batch), int(mem_budget))
+        h = ctypes.c_void_p()
+        st = native().knns_server_create(ctypes.byref(h), corpus._h, ctypes.byref(cfg))
+        if st:
+            _err(st)
+        self._h = h
+        self._out = {}
+
+    def submit(self, queries, k: int) -> int:
+        q = np.ascontiguousarray(queries, dtype=self._dtype)
+        n = q.shape[0]
+        D = np.empty((n, k), dtype=self._dtype)
+        I = np.empty((n, k), dtype=np.int32)
+        t = ctypes.c_ulonglong()
+        st = native().knns_server_submit(self._h, q.ctypes.data, KNNS_F64 if self._dtype == np.float64 else KNNS_F32,
+                                         n, int(k), D.ctypes.data, I.ctypes.data, ctypes.byref(t))
+        if st:
+            _err(st)
+        self._out[t.value] = (D, I)
+        return t.value
+
+    def result(self, ticket: int, timeout: float = -1.0):
+        D, I = self._out.pop(ticket)
+        st = native().knns_server_wait(self._h, ticket, float(timeout))
+        if st:
+            _err(st)
+        return D, I
+
+    def stats(self, reset: bool = False) -> dict:
+        s = ServerStats()
+        st = native().knns_server_stats_get(self._h, ctypes.byref(s), 1 if reset else 0)
+        if st:
+            _err(st)
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            native().knns_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
