@@ -47,6 +47,85 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def run_serving(knns, corpus, Q, seconds: float) -> dict:
+    """Requests of log-uniform size in [1, 4096] queries (drawn from Q) with k uniform in
+    {1, 10, 100}. Phase 1, closed loop: 8 requests kept in flight -> saturation q/s. Phase 2,
+    open loop: Poisson arrivals offering half the saturation rate -> latency percentiles
+    (submit to results ready, measured by the server)."""
+    import threading
+
+    g = np.random.default_rng(9)
+
+    def draw():
+        n = int(min(4096, max(1, round(np.exp(g.uniform(0.0, np.log(4096.0)))))))
+        s0 = int(g.integers(0, Q.shape[0] - n + 1))
+        return Q[s0:s0 + n], int(g.choice((1, 10, 100)))
+
+    with knns.Server(corpus, precision="bf16", max_k=100, max_batch=16384) as srv:
+        for _ in range(3):                        # warm the buckets
+            srv.result(srv.submit(*draw()))
+        srv.stats(reset=True)
+        # phase 1: closed loop
+        inflight, nq = [], 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            while len(inflight) < 8:
+                q, k = draw()
+                inflight.append((srv.submit(q, k), q.shape[0]))
+            t, n = inflight.pop(0)
+            srv.result(t)
+            nq += n
+        for t, n in inflight:
+            srv.result(t)
+            nq += n
+        sat_qps = nq / (time.perf_counter() - t0)
+        sat = srv.stats(reset=True)
+        # phase 2: open loop, Poisson arrivals at half the saturation rate
+        mean_n = (4096 - 1) / np.log(4096.0)
+        rate = 0.5 * sat_qps / mean_n             # requests per second
+        tickets, lock, done = [], threading.Lock(), []
+
+        def waiter():
+            while True:
+                with lock:
+                    item = tickets.pop(0) if tickets else None
+                if item is None:
+                    if done:
+                        return
+                    time.sleep(0.0005)
+                    continue
+                srv.result(item)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        nq2 = 0
+        t0 = time.perf_counter()
+        t_next = t0
+        while True:
+            t_next += g.exponential(1.0 / rate)
+            if t_next - t0 > seconds:
+                break
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            q, k = draw()
+            tk = srv.submit(q, k)
+            nq2 += q.shape[0]
+            with lock:
+                tickets.append(tk)
+        done.append(True)
+        th.join()
+        el = time.perf_counter() - t0
+        st = srv.stats()
+    return {"corpus": "1,000,000 x 128 BF16 (resident), shortlist k+k/2+16, exact FP32 rerank",
+            "requests": "queries per request log-uniform in [1, 4096], k uniform in {1, 10, 100}",
+            "saturation_qps": sat_qps, "saturation_mean_batch": sat["mean_batch"],
+            "saturation_p50_ms": sat["p50_ms"], "saturation_p99_ms": sat["p99_ms"],
+            "offered_qps": 0.5 * sat_qps, "achieved_qps": nq2 / el, "requests_served": st["requests"],
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+            "mean_batch": st["mean_batch"], "batches": st["batches"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +140,9 @@ def main() -> int:
     ap.add_argument("--skip-recall", action="store_true")
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-host-io", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=3.0,
+                    help="duration of each serving phase (saturation, then Poisson at half load)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (elementwise read+write) three times; "
                          "its FETCH_SIZE/WRITE_SIZE counters calibrate the HBM-traffic counters")
@@ -167,6 +249,12 @@ def main() -> int:
         host_io["dropin_heap_call_s"] = t_call
         log(f"host-memory boundary: {host_io}")
 
+    # ---- serving: continuous batching over the resident corpus (knns_server_*) ----
+    serving = None
+    if not args.skip_serving:
+        serving = run_serving(knns, corpus, Q, args.serving_seconds)
+        log(f"serving: {serving}")
+
     # ---- recall vs the FP64 oracle on a query sample ----
     quality = {}
     if not args.skip_recall:
@@ -225,8 +313,11 @@ def main() -> int:
         "fp32_path": fp32,
         "speedup_vs_fp32": (qps / fp32["qps"]) if fp32 else None,
         "host_boundary": host_io,
-        "p50_ms": float(np.percentile(lat_ms, 50)),
-        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "p50_ms": serving["p50_ms"] if serving else None,
+        "p99_ms": serving["p99_ms"] if serving else None,
+        "serving": serving,
+        "step_p50_ms": float(np.percentile(lat_ms, 50)),
+        "step_p99_ms": float(np.percentile(lat_ms, 99)),
         "quality": quality,
         "engine": {**stats, "scan_s": kst["scan_s"], "select_s": kst["select_s"]},
     }
