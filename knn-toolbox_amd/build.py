@@ -67,6 +67,29 @@ def build(verbose: bool = False) -> list[Path]:
         _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", str(INC), *extra, str(PKG / "cli" / "knnTest.cpp"),
               "-L", str(LIB), f"-l{lib}", "-Wl,-rpath,$ORIGIN/../lib", "-o", str(exe)])
         outs.append(exe)
+    # MEX gateways (mex/mexknns.cpp): against the test stand-in of the MEX API always (driven by
+    # tests/test_mex.py), and as real MEX files next to the toolbox .m files when MATLAB's mex or
+    # Octave's mkoctfile is installed
+    shim = PKG.parent / "tests" / "mex_shim"
+    mexdir = LIB / "mex_test"
+    mexdir.mkdir(exist_ok=True)
+    variants = [("mexknnsBitonic", "cuknn", []), ("mexknnsHeap", "cuknn", ["-DKNNS_MEX_HEAP"]),
+                ("mexknnsDBitonic", "cuknn_double", ["-D__DOUBLE__"]),
+                ("mexknnsDHeap", "cuknn_double", ["-D__DOUBLE__", "-DKNNS_MEX_HEAP"])]
+    if shim.exists():
+        for name, lib, extra in variants:
+            so = mexdir / f"{name}_shim.so"
+            _run([cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-I", str(shim), "-I", str(INC), *extra,
+                  str(PKG / "mex" / "mexknns.cpp"), str(shim / "mex_shim.cpp"), "-L", str(LIB), f"-l{lib}",
+                  "-Wl,-rpath,$ORIGIN/..", "-o", str(so)])
+            outs.append(so)
+    mex = shutil.which("mex") or shutil.which("mkoctfile")
+    if mex:
+        for name, lib, extra in variants:
+            cmd = [mex] + (["--mex"] if mex.endswith("mkoctfile") else []) + [
+                f"-I{INC}", *extra, str(PKG / "mex" / "mexknns.cpp"), f"-L{LIB}", f"-l{lib}",
+                "-o", str(PKG / "toolbox" / name)]
+            _run(cmd)
     for o in objs:
         o.unlink(missing_ok=True)
     if verbose:

@@ -59,6 +59,7 @@ double run(knntype* query, knntype* data, knntype* values, knntype* indices, int
     knns_params p;
     knns_params_init(&p, kPrec, k);
     p.select = select;
+    p.flags = KNNS_KERNEL_TIMES;   // knnsGetLastTimes: distance and search stage times
     idx.resize((size_t)numQueries * k);
     st = knns_search(c, query, kDtype, numQueries, &p, values, idx.data());
     knns_last_stats(&stats);
