@@ -102,6 +102,10 @@ knns_status knns_last_stats(knns_stats* out);
 int knns_device_count(void);
 /* releases the cached workspace of the engine */
 void knns_release_workspace(void);
+/* Replaces the planner's tuning table (text: "precision d k n min_waves ratio [ms]" per line,
+ * as written by knn-toolbox_amd/tune.py); NULL or "" clears it (built-in defaults). Without a
+ * call the table is read from $KNNS_TUNING_TABLE or <library dir>/../tuning/gfx950.tsv. */
+knns_status knns_set_tuning_table(const char* path);
 
 /* ---- serving: continuous batching over a resident corpus ----
  * No counterpart in the reference (its API is one synchronous call per query batch); this is

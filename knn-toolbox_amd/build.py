@@ -86,9 +86,10 @@ def build(verbose: bool = False) -> list[Path]:
     mex = shutil.which("mex") or shutil.which("mkoctfile")
     if mex:
         for name, lib, extra in variants:
-            cmd = [mex] + (["--mex"] if mex.endswith("mkoctfile") else []) + [
+            octave = mex.endswith("mkoctfile")
+            cmd = [mex] + (["--mex"] if octave else []) + [
                 f"-I{INC}", *extra, str(PKG / "mex" / "mexknns.cpp"), f"-L{LIB}", f"-l{lib}",
-                "-o", str(PKG / "toolbox" / name)]
+                "-o" if octave else "-output", str(PKG / "toolbox" / name)]
             _run(cmd)
     for o in objs:
         o.unlink(missing_ok=True)

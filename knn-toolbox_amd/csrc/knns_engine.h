@@ -43,8 +43,17 @@ struct Plan {
   std::vector<Round> rounds;
 };
 
+// Planner knobs measured by the autotuner (knn-toolbox_amd/tune.py) and persisted as a table.
+struct Tuning {
+  int min_waves = 5;   // slices: the smallest count filling >= min_waves waves of resident WGs
+  int ratio = 0;       // corpus growth between threshold rounds; 0 = x4 below m/16, then x2
+};
+// Nearest table entry for (precision, d, k, n) (log distance over d, k, n); defaults without a
+// table or without an entry of that precision.
+Tuning tuning_for(int prec, int d, int k, long n);
+
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
-               int slices_req);
+               int slices_req, int min_waves = 5);
 
 // Device buffers of one search.
 struct SearchBuffers {

@@ -12,6 +12,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+
+#include <dlfcn.h>
 #include <string>
 #include <vector>
 
@@ -134,6 +136,93 @@ Engine& engine() {
 
 }  // namespace
 
+// ---- tuning table ----
+// Text file, one entry per line: precision d k n min_waves ratio [measured ms]; '#' starts a
+// comment. Loaded at the first search from $KNNS_TUNING_TABLE, else from
+// <directory of libknns_amd.so>/../tuning/gfx950.tsv; knns_set_tuning_table replaces it.
+namespace {
+struct TuneEntry {
+  int prec, d, k;
+  long n;
+  Tuning t;
+};
+std::mutex g_tune_mu;
+std::vector<TuneEntry> g_tune;
+bool g_tune_loaded = false;
+
+knns_status load_tuning(const char* path, std::vector<TuneEntry>* out) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(KNNS_ERR_INVALID, "cannot open tuning table %s", path);
+  std::vector<TuneEntry> v;
+  char line[512];
+  int ln = 0;
+  while (std::fgets(line, sizeof(line), f)) {
+    ++ln;
+    char* h = std::strchr(line, '#');
+    if (h) *h = '\0';
+    char* p = line;
+    while (*p == ' ' || *p == '\t') ++p;
+    if (*p == '\0' || *p == '\n' || *p == '\r') continue;
+    char prec[16];
+    TuneEntry e{};
+    if (std::sscanf(p, "%15s %d %d %ld %d %d", prec, &e.d, &e.k, &e.n, &e.t.min_waves, &e.t.ratio) != 6) {
+      std::fclose(f);
+      return fail(KNNS_ERR_INVALID, "%s:%d: expected 'precision d k n min_waves ratio'", path, ln);
+    }
+    if (!std::strcmp(prec, "fp32")) e.prec = KNNS_PREC_FP32;
+    else if (!std::strcmp(prec, "bf16")) e.prec = KNNS_PREC_BF16;
+    else if (!std::strcmp(prec, "fp8")) e.prec = KNNS_PREC_FP8;
+    else if (!std::strcmp(prec, "fp64")) e.prec = KNNS_PREC_FP64;
+    else {
+      std::fclose(f);
+      return fail(KNNS_ERR_INVALID, "%s:%d: unknown precision '%s'", path, ln, prec);
+    }
+    if (e.d < 1 || e.k < 1 || e.n < 1 || e.t.min_waves < 1 || e.t.min_waves > 64 ||
+        (e.t.ratio != 0 && (e.t.ratio < 2 || e.t.ratio > 16))) {
+      std::fclose(f);
+      return fail(KNNS_ERR_INVALID, "%s:%d: value out of range", path, ln);
+    }
+    v.push_back(e);
+  }
+  std::fclose(f);
+  *out = std::move(v);
+  return KNNS_OK;
+}
+
+std::string default_tuning_path() {
+  if (const char* env = std::getenv("KNNS_TUNING_TABLE")) return env;
+  Dl_info info;
+  if (dladdr((void*)&default_tuning_path, &info) && info.dli_fname) {
+    std::string lib = info.dli_fname;
+    const size_t slash = lib.rfind('/');
+    const std::string dir = slash == std::string::npos ? "." : lib.substr(0, slash);
+    return dir + "/../tuning/gfx950.tsv";
+  }
+  return "";
+}
+}  // namespace
+
+Tuning tuning_for(int prec, int d, int k, long n) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tune_loaded) {
+    g_tune_loaded = true;
+    const std::string path = default_tuning_path();
+    std::vector<TuneEntry> v;
+    const std::string keep = g_err;   // a missing default table is not an error of the caller
+    if (!path.empty() && load_tuning(path.c_str(), &v) == KNNS_OK) g_tune = std::move(v);
+    g_err = keep;
+  }
+  Tuning best;
+  double bd = 1e30;
+  for (const TuneEntry& e : g_tune) {
+    if (e.prec != prec) continue;
+    const double dist = std::fabs(std::log2((double)d / e.d)) + std::fabs(std::log2((double)k / e.k)) +
+                        std::fabs(std::log2((double)n / e.n));
+    if (dist < bd) { bd = dist; best = e.t; }
+  }
+  return best;
+}
+
 // padded dimension and MFMA chunk for a dimension d
 void dims_for(int d, int* dp, int* dc) {
   const int d16 = (int)round_up(d, 16);
@@ -153,7 +242,7 @@ int default_shortlist(int prec, int k) {
 }
 
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
-               int slices_req) {
+               int slices_req, int min_waves) {
   Plan P{};
   dims_for(d, &P.dp, &P.dc);
   P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : PREC_F32;
@@ -165,9 +254,9 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   P.cmax = cmax;
   // scan workgroups: 4 query groups each, 2 resident per CU. Choose the slice count so that the
   // workgroups of a large round fill whole waves of resident workgroups (a last wave that is
-  // mostly empty leaves the chip idle): the smallest S with >= 5 waves and >= 95% of the last
-  // wave used, else the S with the best fill (measured on 1M x 128, 10k queries: 5 waves beat 4
-  // by 5%, more waves add merge sources for no gain).
+  // mostly empty leaves the chip idle): the smallest S with >= min_waves waves and >= 95% of the
+  // last wave used, else the S with the best fill. min_waves comes from the tuning table
+  // (default 5: on 1M x 128 with 10k queries 5 waves beat 4 by 5%, more add merge sources).
   const long wg_rows = P.groups / 4;
   const long cap = (long)cus * 2;
   int smax = 1;
@@ -177,8 +266,8 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
       const long wgs = wg_rows * S;
       const long waves = (wgs + cap - 1) / cap;
       const double fill = (double)wgs / (double)(waves * cap);
-      if (waves >= 5 && fill >= 0.95) { smax = S; best = 2.0; break; }
-      const double score = fill * std::min<double>(1.0, (double)wgs / (4.0 * cap));
+      if (waves >= min_waves && fill >= 0.95) { smax = S; best = 2.0; break; }
+      const double score = fill * std::min<double>(1.0, (double)wgs / ((double)min_waves * cap));
       if (score > best) { best = score; smax = S; }
     }
   }
@@ -421,7 +510,9 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     }
   }
 
-  Plan P = make_plan(m, c->d, n, kmax, cmax, prec, E.cus, p->ratio, p->slices);
+  const Tuning tu = tuning_for(prec, c->d, kmax, n);
+  Plan P = make_plan(m, c->d, n, kmax, cmax, prec, E.cus, p->ratio > 0 ? p->ratio : tu.ratio, p->slices,
+                     tu.min_waves);
   if (P.dp != c->dp) return fail(KNNS_ERR_INVALID, "corpus/plan dimension mismatch");
 
   // workspace (grow-only engine slots)
@@ -682,7 +773,8 @@ knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns
   if (!out || m < 1 || d < 1 || n < 1 || k < 1 || k > 1024)
     return fail(KNNS_ERR_INVALID, "bad plan arguments");
   const int cl = std::min<long>(std::min(default_shortlist(precision, k), 2048), m);
-  Plan P = make_plan(m, d, n, k, cl, precision, 256, 0, 0);
+  const Tuning tu = tuning_for(precision, d, k, n);
+  Plan P = make_plan(m, d, n, k, cl, precision, 256, tu.ratio, 0, tu.min_waves);
   std::memset(out, 0, sizeof(*out));
   out->rounds = (long)P.rounds.size();
   out->groups = P.groups;
@@ -690,6 +782,18 @@ knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns
   out->qb = P.qb;
   out->dc = P.dc;
   out->dp = P.dp;
+  return KNNS_OK;
+}
+
+knns_status knns_set_tuning_table(const char* path) {
+  g_err.clear();
+  std::vector<TuneEntry> v;
+  if (path && path[0]) {
+    if (knns_status st = load_tuning(path, &v)) return st;
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune = std::move(v);
+  g_tune_loaded = true;
   return KNNS_OK;
 }
 

@@ -249,7 +249,8 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
   long budget = cfg->mem_budget;
   std::vector<knns::Plan> plans;
   for (long cap : caps) {
-    knns::Plan P = knns::make_plan(c->m, c->d, cap, cfg->max_k, S->cmax, prec, cus, 0, 0);
+    const knns::Tuning tu = knns::tuning_for(prec, c->d, cfg->max_k, cap);
+    knns::Plan P = knns::make_plan(c->m, c->d, cap, cfg->max_k, S->cmax, prec, cus, tu.ratio, 0, tu.min_waves);
     const knns::SearchSizes y = knns::search_sizes(c, P, cap, cap * (long)cfg->max_k, false, S->osz == 8);
     knns::SearchSizes t = z;
 #define SRV_MAX(f) t.f = std::max(t.f, y.f)

@@ -114,6 +114,8 @@ def native():
         L.knns_device_count.restype = I
         L.knns_release_workspace.argtypes = []
         L.knns_release_workspace.restype = None
+        L.knns_set_tuning_table.argtypes = [ctypes.c_char_p]
+        L.knns_set_tuning_table.restype = I
         SC, SS = ctypes.POINTER(ServerConfig), ctypes.POINTER(ServerStats)
         L.knns_server_create.argtypes = [ctypes.POINTER(V), V, SC]
         L.knns_server_create.restype = I
