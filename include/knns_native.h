@@ -117,7 +117,7 @@ knns_status knns_set_tuning_table(const char* path);
 typedef struct knns_server_s* knns_server;
 
 typedef struct {
-  int precision;       /* knns_precision (FP32, BF16; FP64 for double corpora) */
+  int precision;       /* knns_precision (FP32, BF16, FP8; FP64 for double corpora) */
   int max_k;           /* largest k a request may ask for, 1..1024 */
   long max_batch;      /* query slots of the largest bucket (>= 1) */
   long mem_budget;     /* device bytes the server may hold; 0 = no limit. Buckets whose workspace

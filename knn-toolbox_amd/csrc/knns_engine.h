@@ -20,6 +20,9 @@ struct knns_corpus_s {
   float* x32 = nullptr;
   uint16_t* x16 = nullptr;
   double* x64 = nullptr;
+  uint8_t* x8 = nullptr;   // FP8 (E4M3) rows with their scale, rows_pad x ld8 bytes
+  int dp8 = 0;             // FP8 row bytes (d rounded up to 64; at most 256)
+  long ld8 = 0;            // dp8 + 16 (the augmentation unit)
 };
 
 namespace knns {
@@ -30,6 +33,7 @@ knns_status fail(knns_status st, const char* fmt, ...);
 
 inline long round_up(long x, long a) { return (x + a - 1) / a * a; }
 void dims_for(int d, int* dp, int* dc);
+int fp8_row_bytes(int d);
 int default_shortlist(int prec, int k);
 
 struct Round {
@@ -38,6 +42,7 @@ struct Round {
   int dense;   // every row is a candidate (threshold +inf): unfiltered dense writes
 };
 
+// dp: padded FP32 dims of the corpus; dc: the scan's chunk (FP8: the FP8 row bytes)
 struct Plan {
   int prec_kernel, qb, dc, dp, groups, n_pad, slices_max, sub_cap, kmax, cmax;
   std::vector<Round> rounds;
@@ -59,6 +64,7 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
 struct SearchBuffers {
   void* q32 = nullptr;      // packed FP32 query rows, n_pad x ld32
   void* q16 = nullptr;      // packed BF16 query rows (BF16 plans)
+  void* q8 = nullptr;       // packed FP8 query rows (FP8 plans)
   void* q64 = nullptr;      // FP64 query master (double corpora)
   void* thr = nullptr;      // u32 per query slot
   void* kc = nullptr;       // shortlist per query slot
@@ -76,7 +82,7 @@ struct SearchBuffers {
 
 // Byte sizes of the buffers above for a plan (q_slots = query slots of the search).
 struct SearchSizes {
-  size_t q32, q16, q64, thr, kc, kq, off, self, sub, subcnt, carry, ccnt, flags, odist, oidx;
+  size_t q32, q16, q8, q64, thr, kc, kq, off, self, sub, subcnt, carry, ccnt, flags, odist, oidx;
 };
 SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long total_out, bool excl,
                          bool out64);

@@ -44,7 +44,7 @@ namespace {
   } while (0)
 
 enum Slot {
-  W_Q32, W_Q16, W_Q64, W_STAGE, W_THR, W_KC, W_KQ, W_OFF, W_SELF, W_SUB, W_SUBCNT, W_CARRY0,
+  W_Q32, W_Q16, W_Q8, W_Q64, W_STAGE, W_THR, W_KC, W_KQ, W_OFF, W_SELF, W_SUB, W_SUBCNT, W_CARRY0,
   W_CARRY1, W_CCNT0, W_CCNT1, W_FLAGS, W_ODIST, W_OIDX, W_NSLOTS
 };
 
@@ -223,6 +223,9 @@ Tuning tuning_for(int prec, int d, int k, long n) {
   return best;
 }
 
+// FP8 operand row: d bytes rounded up to whole 64-byte MFMA steps
+int fp8_row_bytes(int d) { return (int)round_up(d, 64); }
+
 // padded dimension and MFMA chunk for a dimension d
 void dims_for(int d, int* dp, int* dc) {
   const int d16 = (int)round_up(d, 16);
@@ -245,8 +248,9 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
                int slices_req, int min_waves) {
   Plan P{};
   dims_for(d, &P.dp, &P.dc);
-  P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : PREC_F32;
-  P.qb = (P.prec_kernel == PREC_BF16 && n > 32) ? 2 : 1;
+  P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : (prec == KNNS_PREC_FP8 ? PREC_FP8 : PREC_F32);
+  if (P.prec_kernel == PREC_FP8) P.dc = fp8_row_bytes(d);
+  P.qb = (P.prec_kernel != PREC_F32 && n > 32) ? 2 : 1;
   const int qw = 32 * P.qb;
   P.groups = (int)round_up((n + qw - 1) / qw, 4);  // a scan workgroup runs 4 query groups
   P.n_pad = P.groups * qw;
@@ -311,6 +315,7 @@ SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long tot
   SearchSizes z{};
   z.q32 = (size_t)n_pad * c->ld32 * 4;
   z.q16 = P.prec_kernel == PREC_BF16 ? (size_t)n_pad * c->ld16 * 2 : 0;
+  z.q8 = P.prec_kernel == PREC_FP8 ? (size_t)n_pad * c->ld8 : 0;
   z.q64 = c->dtype == KNNS_F64 ? (size_t)n * c->d * 8 : 0;
   z.thr = (size_t)n_pad * 4;
   z.kc = (size_t)n_pad * 4;
@@ -351,14 +356,21 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   pa.dp = c->dp;
   pa.is_query = 1;
   KNNS_TRY(launch_pack(pa, st));
+  if (P.prec_kernel == PREC_FP8) {
+    pa.x8 = (uint8_t*)B.q8;
+    pa.ld8 = c->ld8;
+    pa.dp8 = c->dp8;
+    KNNS_TRY(launch_pack_fp8(pa, st));
+  }
   KNNS_TRY(launch_init_queries((uint32_t*)B.thr, (int*)B.kc, (int*)B.kq, (long*)B.off,
                                (int*)B.self, (int*)B.ccnt[0], n, n_pad, L.uniform ? L.kuni : 1,
                                L.cmax, L.self_offset, m, L.n_dev, L.uniform ? 1 : 0, st));
   KNNS_TRY(launch_fill_i32((int*)B.flags, 0, 16, st));
 
-  const void* xop = P.prec_kernel == PREC_BF16 ? (const void*)c->x16 : (const void*)c->x32;
-  const void* qop = P.prec_kernel == PREC_BF16 ? B.q16 : B.q32;
-  const long ldx = P.prec_kernel == PREC_BF16 ? c->ld16 : c->ld32;
+  const int pk = P.prec_kernel;
+  const void* xop = pk == PREC_BF16 ? (const void*)c->x16 : (pk == PREC_FP8 ? (const void*)c->x8 : (const void*)c->x32);
+  const void* qop = pk == PREC_BF16 ? B.q16 : (pk == PREC_FP8 ? B.q8 : B.q32);
+  const long ldx = pk == PREC_BF16 ? c->ld16 : (pk == PREC_FP8 ? c->ld8 : c->ld32);
   int cur = 0;
   for (size_t r = 0; r < P.rounds.size(); ++r) {
     const Round& R = P.rounds[r];
@@ -367,7 +379,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     sa.Qp = qop;
     sa.ldx = ldx;
     sa.ldq = ldx;
-    sa.dp = P.dp;
+    sa.dp = pk == PREC_FP8 ? c->dp8 : P.dp;
     sa.row_begin = R.begin;
     sa.row_end = R.end;
     sa.slice_rows = R.slice_rows;
@@ -462,14 +474,14 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     return fail(KNNS_ERR_INVALID, "query dtype %d does not match corpus dtype %d", dtype, c->dtype);
   if (n > (1L << 30)) return fail(KNNS_ERR_INVALID, "too many queries (%ld)", n);
   const int prec = p->precision;
-  if (prec == KNNS_PREC_FP8)
-    return fail(KNNS_ERR_UNSUPPORTED, "FP8 shortlist is not available in this build");
-  if (prec != KNNS_PREC_FP32 && prec != KNNS_PREC_BF16 && prec != KNNS_PREC_FP64)
+  if (prec != KNNS_PREC_FP32 && prec != KNNS_PREC_FP8 && prec != KNNS_PREC_BF16 && prec != KNNS_PREC_FP64)
     return fail(KNNS_ERR_INVALID, "unknown precision %d", prec);
   if (prec == KNNS_PREC_FP64 && c->dtype != KNNS_F64)
     return fail(KNNS_ERR_INVALID, "FP64 precision needs a double corpus");
   if (prec == KNNS_PREC_BF16 && !c->x16)
     return fail(KNNS_ERR_INVALID, "BF16 precision needs a corpus built with KNNS_CORPUS_BF16");
+  if (prec == KNNS_PREC_FP8 && !c->x8)
+    return fail(KNNS_ERR_INVALID, "FP8 precision needs a corpus built with KNNS_CORPUS_FP8");
   const bool dev = (p->flags & KNNS_DEVICE_PTRS) != 0;
   const bool excl = (p->flags & KNNS_EXCLUDE_SELF) != 0;
   const bool out64 = c->dtype == KNNS_F64 || (p->flags & KNNS_DIST_F64);
@@ -523,6 +535,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   if ((sst = E.get(slot, (size_t)(bytes), &ptr)) != KNNS_OK) return sst
   KNNS_WS(W_Q32, z.q32, B.q32);
   if (z.q16) KNNS_WS(W_Q16, z.q16, B.q16);
+  if (z.q8) KNNS_WS(W_Q8, z.q8, B.q8);
   if (z.q64) KNNS_WS(W_Q64, z.q64, B.q64);
   KNNS_WS(W_THR, z.thr, B.thr);
   KNNS_WS(W_KC, z.kc, B.kc);
@@ -666,8 +679,8 @@ knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, lo
   if (m < 1 || m > 0x7fffff00L) return fail(KNNS_ERR_INVALID, "corpus rows %ld out of range", m);
   if (d < 1 || d > 65536) return fail(KNNS_ERR_INVALID, "dimension %d out of range", d);
   if (dtype != KNNS_F32 && dtype != KNNS_F64) return fail(KNNS_ERR_INVALID, "bad dtype %d", dtype);
-  if (flags & KNNS_CORPUS_FP8)
-    return fail(KNNS_ERR_UNSUPPORTED, "FP8 corpus copy is not available in this build");
+  if ((flags & KNNS_CORPUS_FP8) && fp8_row_bytes(d) > 256)
+    return fail(KNNS_ERR_UNSUPPORTED, "FP8 corpus copy supports d <= 256 (d=%d)", d);
   if (knns_status st = E.init()) return st;
   knns_corpus c = new knns_corpus_s();
   c->m = m;
@@ -678,9 +691,14 @@ knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, lo
   c->rows_pad = round_up(m, 128);  // whole scan stages (64 or 128 rows) never leave the padding
   c->ld32 = c->dp + 4;
   c->ld16 = c->dp + 16;
+  if (flags & KNNS_CORPUS_FP8) {
+    c->dp8 = fp8_row_bytes(d);
+    c->ld8 = c->dp8 + 16;
+  }
   auto cleanup = [&](knns_status st) {
     if (c->x32) (void)hipFree(c->x32);
     if (c->x16) (void)hipFree(c->x16);
+    if (c->x8) (void)hipFree(c->x8);
     if (c->x64) (void)hipFree(c->x64);
     delete c;
     (void)hipGetLastError();
@@ -691,6 +709,8 @@ knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, lo
   if ((flags & KNNS_CORPUS_BF16) &&
       hipMalloc((void**)&c->x16, (size_t)c->rows_pad * c->ld16 * 2) != hipSuccess)
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus BF16 rows"));
+  if ((flags & KNNS_CORPUS_FP8) && hipMalloc((void**)&c->x8, (size_t)c->rows_pad * c->ld8) != hipSuccess)
+    return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP8 rows"));
   if (dtype == KNNS_F64 && hipMalloc((void**)&c->x64, (size_t)m * d * 8) != hipSuccess)
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP64 master"));
   const size_t esz = dtype == KNNS_F64 ? 8 : 4;
@@ -725,7 +745,10 @@ knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, lo
     pa.x64 = c->x64 ? c->x64 + (size_t)r0 * d : nullptr;
     pa.dp = c->dp;
     pa.is_query = 0;
-    if (launch_pack(pa, st) != hipSuccess) {
+    pa.x8 = c->x8 ? c->x8 + (size_t)r0 * c->ld8 : nullptr;
+    pa.ld8 = c->ld8;
+    pa.dp8 = c->dp8;
+    if (launch_pack(pa, st) != hipSuccess || (c->x8 && launch_pack_fp8(pa, st) != hipSuccess)) {
       if (stage) (void)hipFree(stage);
       return cleanup(fail(KNNS_ERR_DEVICE, "corpus pack launch failed"));
     }
@@ -749,6 +772,7 @@ knns_status knns_corpus_destroy(knns_corpus c) {
   std::lock_guard<std::recursive_mutex> lock(E.mu);
   if (c->x32) (void)hipFree(c->x32);
   if (c->x16) (void)hipFree(c->x16);
+  if (c->x8) (void)hipFree(c->x8);
   if (c->x64) (void)hipFree(c->x64);
   delete c;
   return KNNS_OK;

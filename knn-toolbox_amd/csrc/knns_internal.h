@@ -107,10 +107,16 @@ struct PackArgs {
   double* x64;           // optional double master copy [rows][d]
   int dp;                // padded dims
   int is_query;          // aug = ones instead of -||r||^2/2
+  uint8_t* x8;           // optional FP8 rows [rows_pad][ld8 bytes] (pack_fp8)
+  long ld8;              // bytes per FP8 row: dp8 + 16
+  int dp8;               // FP8 padded dims (multiple of 64)
 };
 
 // Launchers (knns_kernels.hip). All enqueue on `stream` and return hipError_t.
 hipError_t launch_pack(const PackArgs& a, hipStream_t stream);
+// FP8 rows: [dp8 bytes E4M3 of r / 2^e_r][aug unit: 3 BF16 terms of -||r~||^2 / 2 (r~ = the
+// dequantised row), 2 zero bytes, e_r + 127 (E8M0), 7 zero bytes]; query rows carry zero terms.
+hipError_t launch_pack_fp8(const PackArgs& a, hipStream_t stream);
 hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, hipStream_t stream);
 hipError_t launch_merge(const MergeArgs& a, hipStream_t stream);
 hipError_t launch_final(const FinalArgs& a, hipStream_t stream);

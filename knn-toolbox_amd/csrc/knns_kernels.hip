@@ -252,6 +252,87 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// FP8 (OCP E4M3: 1 sign, 4 exponent bits with bias 7, 3 mantissa bits; max 448, no infinities)
+// ------------------------------------------------------------------------------------------------
+// Round to nearest even, saturating at +-448 (inputs are scaled into range first).
+__device__ __forceinline__ uint8_t f32_to_e4m3(float x) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, x);
+  const uint8_t sgn = (uint8_t)((u >> 24) & 0x80u);
+  const float ax = __builtin_fabsf(x);
+  if (ax >= 448.0f) return sgn | 0x7Eu;
+  if (ax < 0.015625f) {                       // below 2^-6: subnormals k * 2^-9, k = 0..8
+    const int kq = (int)__builtin_rintf(ax * 512.0f);
+    return sgn | (uint8_t)kq;                 // k = 8 is the smallest normal, 0x08
+  }
+  const uint32_t ua = __builtin_bit_cast(uint32_t, ax);
+  uint32_t r = ua + 0x7FFFFu + ((ua >> 20) & 1u);   // keep 3 mantissa bits, ties to even
+  r &= 0xFFF00000u;
+  const int e = (int)((r >> 23) & 0xFF) - 127;
+  const uint32_t mnt = (r >> 20) & 7u;
+  if (e > 8 || (e == 8 && mnt == 7u)) return sgn | 0x7Eu;
+  return sgn | (uint8_t)(((uint32_t)(e + 7) << 3) | mnt);
+}
+__device__ __forceinline__ double e4m3_to_f64(uint8_t b) {
+  const int ex = (b >> 3) & 15, mnt = b & 7;
+  const double v = ex == 0 ? (double)mnt * 0x1p-9 : (1.0 + mnt / 8.0) * __builtin_ldexp(1.0, ex - 7);
+  return (b & 0x80) ? -v : v;
+}
+// smallest power-of-two exponent p with amax / 2^p <= 448 (0 for amax = 0), clamped to E8M0
+__device__ __forceinline__ int fp8_scale_exp(double amax) {
+  if (!(amax > 0.0)) return 0;
+  const uint64_t ub = __builtin_bit_cast(uint64_t, amax / 448.0);   // ceil(log2) from the bits
+  const int eb = (int)((ub >> 52) & 0x7FF) - 1023;
+  int p = (ub & ((1ull << 52) - 1)) ? eb + 1 : eb;
+  while (amax / __builtin_ldexp(1.0, p) > 448.0) ++p;
+  while (p > -126 && amax / __builtin_ldexp(1.0, p - 1) <= 448.0) --p;
+  return p < -126 ? -126 : (p > 126 ? 126 : p);
+}
+
+// One wave per row: wave max -> scale, quantise, FP64 norm of the dequantised row, aug unit.
+__global__ __launch_bounds__(256) void pack_fp8_kernel(PackArgs a) {
+  const int lane = threadIdx.x & 63;
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= a.rows_pad) return;
+  const bool real = row < a.rows;
+  auto val = [&](int j) -> double {
+    if (!real || j >= a.d) return 0.0;
+    return a.src_f64 ? ((const double*)a.src)[row * a.ld_src + j]
+                     : (double)((const float*)a.src)[row * a.ld_src + j];
+  };
+  double amax = 0.0;
+  for (int j = lane; j < a.d; j += 64) amax = __builtin_fmax(amax, __builtin_fabs((double)(float)val(j)));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) amax = __builtin_fmax(amax, __shfl_xor(amax, o, 64));
+  const int pe = fp8_scale_exp(amax);
+  const double sc = __builtin_ldexp(1.0, pe), inv = __builtin_ldexp(1.0, -pe);
+  uint8_t* out = a.x8 + row * a.ld8;
+  double nrm = 0.0;
+  for (int j = lane; j < a.dp8; j += 64) {
+    const float f = (float)val(j);
+    const uint8_t b = f32_to_e4m3((float)((double)f * inv));
+    out[j] = b;
+    const double dq = e4m3_to_f64(b) * sc;
+    nrm += dq * dq;
+  }
+  nrm = wave_sum_f64(nrm);
+  if (lane < 4) {
+    // aug unit, as 4 dwords: the BF16 terms (t1, t2), (t3, 0) of -||r~||^2 / 2 (corpus rows;
+    // zeros for queries), then e + 127 in byte 8, then zeros
+    uint32_t w = 0;
+    if (lane < 2 && !a.is_query && real) {
+      const double t = -0.5 * nrm;
+      const uint16_t t1 = f2bf((float)t);
+      const double r1 = t - (double)bf2f(t1);
+      const uint16_t t2 = f2bf((float)r1);
+      const uint16_t t3 = f2bf((float)(r1 - (double)bf2f(t2)));
+      w = lane == 0 ? ((uint32_t)t1 | ((uint32_t)t2 << 16)) : (uint32_t)t3;
+    }
+    if (lane == 2) w = (uint32_t)(pe + 127);
+    *(uint32_t*)(out + a.dp8 + 4 * lane) = w;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
 // scan kernel
 // ------------------------------------------------------------------------------------------------
 // One wave owns QB blocks of 32 queries (MFMA columns) and one corpus slice; it streams the
@@ -658,13 +739,13 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// BF16 single-chunk scan (DP = DC in {32, 64, 128}): the headline configuration
+// BF16 single-chunk scan (DP = DC in {32, 64, 128}, the headline configuration) and FP8 scan
 // ------------------------------------------------------------------------------------------------
 // Same work split and filter as scan_kernel; the stage layout is built so that the loop spends
 // almost no VALU on addressing:
-//   * an LDS row holds only the SU = DC/8 16-byte units of main dims (no padding); unit u of row
-//     r sits at position u ^ sw(r), sw(r) = (r / (16/SU)) % SU, so the 16-byte MFMA operand reads
-//     of 16 consecutive rows (8 with 32 banks) touch distinct banks;
+//   * an LDS row holds only the SU 16-byte units of main dims (no padding); unit u of row r sits
+//     at position u ^ sw(r), sw(r) = (r / (16/SU)) % SU, so the 16-byte MFMA operand reads of 16
+//     consecutive rows (8 with 32 banks) touch distinct banks;
 //   * the augmentation units (16 bytes per row) go to a separate LDS array;
 //   * a wave-wide DMA load fills 64 consecutive units = 64/SU whole rows, so with load L of wave
 //     wv being L = wv + 4l, a lane's source offset inside the stage is the same for every load
@@ -672,24 +753,33 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
 //   * operand reads for step j are at (A0 ^ 16j) + immediate, one XOR per step;
 //   * the first MFMA of each accumulator is the augmentation step with a zero C operand, so no
 //     accumulator is ever cleared, and operands are read one step ahead of their MFMAs.
-template <int DC>
+// BF16 rows (SU = dp/8): one 32x32x16 MFMA per unit pair; lane half h takes units h*SU/2 + j.
+// FP8 rows (OCP E4M3, SU = dp8/16): one scaled 32x32x64 MFMA per 4 units; lane half h takes units
+// h*SU/2 + 2j and + 2j + 1 (32 bytes). The E8M0 row scales (2^e_r of the corpus row, 2^e_q of the
+// query) are the MFMA's per-lane scale operands, so the products are those of the dequantised
+// values r~ = 2^e_r * e4m3(r / 2^e_r). The augmentation stays a BF16 MFMA against the 3 BF16 terms
+// of -||r~||^2 / 2 (half the issue cost of an FP8 one); its unit's byte 8 holds e_r + 127, which the
+// BF16 step multiplies by zero.
+template <int SU_>
 struct SwzGeom {
-  static constexpr int SU = DC / 8;              // 16-byte units of main dims per row
+  static constexpr int SU = SU_;                 // 16-byte units of main dims per row
   static constexpr int SR = 128;                 // rows per stage
   static constexpr int RPL = 64 / SU;            // rows per wave-wide DMA load
   static constexpr int MAIN_LOADS = SR / RPL;    // per stage
   static constexpr int PER_WAVE = MAIN_LOADS / 4;
   static constexpr int MAIN_BYTES = SR * SU * 16;
   static constexpr int AUG_BYTES = SR * 16;
-  static_assert(SU == 4 || SU == 8 || SU == 16, "single-chunk widths 32, 64, 128");
+  static_assert(SU == 4 || SU == 8 || SU == 16, "rows of 64, 128 or 256 bytes");
   static_assert(MAIN_LOADS % 4 == 0, "whole loads per wave");
 };
 
-template <int QB, int DC>
-__global__ __launch_bounds__(256, 2) void scan_bf16_kernel(ScanArgs a) {
-  using G = SwzGeom<DC>;
-  constexpr int SU = G::SU;
-  constexpr int J = DC / 16;                     // MFMA k-steps per row block (one per unit pair)
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+template <int QB, int SU, bool F8>
+__global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
+  using G = SwzGeom<SU>;
+  constexpr int ESZ = F8 ? 1 : 2;                // bytes per element
+  constexpr int J = F8 ? SU / 4 : SU / 2;        // MFMA k-steps per row block
   __shared__ __attribute__((aligned(1024))) uint8_t smem[2 * G::MAIN_BYTES + 2 * G::AUG_BYTES];
   __shared__ __attribute__((aligned(16))) uint32_t kth_lds[4][256 + 32];
   const int tid = threadIdx.x;
@@ -724,12 +814,12 @@ __global__ __launch_bounds__(256, 2) void scan_bf16_kernel(ScanArgs a) {
   }
   const int nst = c0 < c1 ? (c1 - c0 + G::SR - 1) / G::SR : 0;
   const uint8_t* X = (const uint8_t*)a.X;
-  const uint32_t ldb = (uint32_t)a.ldx * 2u;     // row stride in bytes
+  const uint32_t ldb = (uint32_t)a.ldx * ESZ;    // row stride in bytes
 
   // ---- DMA source offsets (bytes, relative to the first row of the load) ----
   const int swl = (4 * wv + (lane >> 4)) % SU;
   const uint32_t dma_off = (uint32_t)(lane / SU) * ldb + (uint32_t)((lane % SU) ^ swl) * 16u;
-  const uint32_t aug_off = (uint32_t)lane * ldb + (uint32_t)a.dp * 2u;
+  const uint32_t aug_off = (uint32_t)lane * ldb + (uint32_t)a.dp * ESZ;
   typedef __attribute__((address_space(3))) uint8_t lds_u8;
   auto issue = [&](int t0, int buf) {
     // running scalar source and LDS pointers (the empty asm keeps them in SGPRs and stops the
@@ -754,14 +844,26 @@ __global__ __launch_bounds__(256, 2) void scan_bf16_kernel(ScanArgs a) {
   };
 
   // ---- per-wave query operand (B), resident for the whole slice ----
-  bf16x8 bq[QB][J];
+  typedef typename std::conditional<F8, i32x8, bf16x8>::type OT;
+  OT bq[QB][J];
+  int sq[QB];                                    // FP8: E8M0 scale byte of the lane's query
   {
-    const uint16_t* Q = (const uint16_t*)a.Qp;
+    const uint8_t* Q = (const uint8_t*)a.Qp;
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
-      const uint16_t* qp = Q + (long)((g * QB + qb) * 32 + col) * a.ldq + h * (DC / 2);
+      const uint8_t* qrow = Q + (size_t)((g * QB + qb) * 32 + col) * a.ldq * ESZ;
+      const uint8_t* qp = qrow + h * (SU / 2) * 16;
 #pragma unroll
-      for (int j = 0; j < J; ++j) bq[qb][j] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 8 * j));
+      for (int j = 0; j < J; ++j) {
+        if constexpr (F8) {
+          const u32x4 lo = *(const u32x4*)(qp + 32 * j), hi = *(const u32x4*)(qp + 32 * j + 16);
+          bq[qb][j] = (i32x8){(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3],
+                              (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+        } else {
+          bq[qb][j] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 16 * j));
+        }
+      }
+      sq[qb] = F8 ? (int)(*(const uint32_t*)(qrow + a.dp + 8) & 0xFFu) : 0;
     }
   }
   bf16x8 baug;
@@ -791,27 +893,47 @@ __global__ __launch_bounds__(256, 2) void scan_bf16_kernel(ScanArgs a) {
       __builtin_amdgcn_sched_barrier(0);
       const uint8_t* m0 = mb + sub * 64 * SU * 16;
       const uint8_t* m1 = m0 + 32 * SU * 16;
-      auto rd = [&](const uint8_t* p, uint32_t off) -> bf16x8 {
-        return __builtin_bit_cast(bf16x8, *(const u32x4*)(p + off));
+      auto rd = [&](const uint8_t* p, uint32_t off) -> u32x4 { return *(const u32x4*)(p + off); };
+      // operand of step j: one unit (BF16) or two (FP8; the second at offset ^ 16)
+      auto rdop = [&](const uint8_t* p, uint32_t off) -> OT {
+        if constexpr (F8) {
+          const u32x4 lo = rd(p, off), hi = rd(p, off ^ 16u);
+          return (i32x8){(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3],
+                         (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+        } else {
+          return __builtin_bit_cast(bf16x8, rd(p, off));
+        }
       };
-      bf16x8 x0 = rd(ab + sub * 64 * 16, AG), x1 = rd(ab + sub * 64 * 16 + 32 * 16, AG);
-      bf16x8 y0 = rd(m0, A0), y1 = rd(m1, A0);
+      constexpr uint32_t STEP = F8 ? 32u : 16u;
+      const u32x4 g0 = rd(ab + sub * 64 * 16, AG), g1 = rd(ab + sub * 64 * 16 + 32 * 16, AG);
+      const int s0 = (int)(g0[2] & 0xFFu), s1 = (int)(g1[2] & 0xFFu);
+      OT y0 = rdop(m0, A0), y1 = rdop(m1, A0);
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb) {
-        acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, baug, (f32x16){0}, 0, 0, 0);
-        acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, baug, (f32x16){0}, 0, 0, 0);
+        acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, g0), baug,
+                                                             (f32x16){0}, 0, 0, 0);
+        acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, g1), baug,
+                                                             (f32x16){0}, 0, 0, 0);
       }
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        x0 = y0; x1 = y1;
+        const OT x0 = y0, x1 = y1;
         if (j + 1 < J) {
-          const uint32_t o = A0 ^ (uint32_t)(16 * (j + 1));
-          y0 = rd(m0, o); y1 = rd(m1, o);
+          const uint32_t o = A0 ^ (STEP * (uint32_t)(j + 1));
+          y0 = rdop(m0, o); y1 = rdop(m1, o);
         }
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) {
-          acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, bq[qb][j], acc[qb][0], 0, 0, 0);
-          acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, bq[qb][j], acc[qb][1], 0, 0, 0);
+          if constexpr (F8) {
+            // formats 0, 0 = E4M3 x E4M3; scale bytes in bits 0..7 (op_sel 0)
+            acc[qb][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x0, bq[qb][j], acc[qb][0], 0, 0,
+                                                                         0, s0, 0, sq[qb]);
+            acc[qb][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x1, bq[qb][j], acc[qb][1], 0, 0,
+                                                                         0, s1, 0, sq[qb]);
+          } else {
+            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, bq[qb][j], acc[qb][0], 0, 0, 0);
+            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, bq[qb][j], acc[qb][1], 0, 0, 0);
+          }
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -1376,6 +1498,14 @@ __global__ void fill_i32_kernel(int* p, int v, long n) {
 // ------------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------------
+hipError_t launch_pack_fp8(const PackArgs& a, hipStream_t stream) {
+  const long blocks = (a.rows_pad + 3) / 4;
+  if (blocks <= 0) return hipSuccess;
+  if (!a.x8 || a.dp8 % 64 || a.ld8 != a.dp8 + 16 || a.d > a.dp8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pack_fp8_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
   const long blocks = (a.rows_pad + 3) / 4;
   if (blocks <= 0) return hipSuccess;
@@ -1409,16 +1539,26 @@ bool scan_supported(int prec, int qb, int dc) {
 // Row granularity of the stages: slice and round boundaries must be multiples of it, and the
 // corpus must be padded to it, so a stage never reads past the padded corpus.
 int scan_stage_rows(int prec, int dp, int dc) {
-  return (prec == PREC_BF16 && dp == dc) ? 128 : 64;
+  return ((prec == PREC_BF16 && dp == dc) || prec == PREC_FP8) ? 128 : 64;
 }
 
 hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, hipStream_t stream) {
   const int sr = scan_stage_rows(prec, a.dp, dc);
   if ((a.row_begin % sr) || (a.slice_rows % sr)) return hipErrorInvalidValue;
   if (a.row_end > 0x7fffff80L) return hipErrorInvalidValue;   // int32 row indices
+  if (prec == PREC_FP8) {
+    // dc = dp = FP8 row bytes (64, 128, 256); ScanArgs strides are in bytes
+    if (a.dp != dc) return hipErrorInvalidValue;
+#define KNNS_F8_CASE(B, D) \
+    if (qb == B && dc == D) return scan_grid(scan_swz_kernel<B, D / 16, true>, a, groups, stream);
+    KNNS_F8_CASE(1, 64) KNNS_F8_CASE(1, 128) KNNS_F8_CASE(1, 256)
+    KNNS_F8_CASE(2, 64) KNNS_F8_CASE(2, 128) KNNS_F8_CASE(2, 256)
+#undef KNNS_F8_CASE
+    return hipErrorInvalidValue;
+  }
   if (prec == PREC_BF16 && a.dp == dc && dc != 96 && !std::getenv("KNNS_SCAN_V2")) {
 #define KNNS_SWZ_CASE(B, D) \
-    if (qb == B && dc == D) return scan_grid(scan_bf16_kernel<B, D>, a, groups, stream);
+    if (qb == B && dc == D) return scan_grid(scan_swz_kernel<B, D / 8, false>, a, groups, stream);
     KNNS_SWZ_CASE(1, 32) KNNS_SWZ_CASE(1, 64) KNNS_SWZ_CASE(1, 128)
     KNNS_SWZ_CASE(2, 32) KNNS_SWZ_CASE(2, 64) KNNS_SWZ_CASE(2, 128)
 #undef KNNS_SWZ_CASE

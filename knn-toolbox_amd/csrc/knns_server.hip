@@ -216,11 +216,11 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
     return fail(KNNS_ERR_INVALID, "max_batch=%ld outside [1, 2^24]", cfg->max_batch);
   if (cfg->mem_budget < 0) return fail(KNNS_ERR_INVALID, "negative memory budget");
   const int prec = cfg->precision;
-  if (prec == KNNS_PREC_FP8) return fail(KNNS_ERR_UNSUPPORTED, "FP8 shortlist is not available in this build");
-  if (prec != KNNS_PREC_FP32 && prec != KNNS_PREC_BF16 && prec != KNNS_PREC_FP64)
+  if (prec != KNNS_PREC_FP32 && prec != KNNS_PREC_BF16 && prec != KNNS_PREC_FP8 && prec != KNNS_PREC_FP64)
     return fail(KNNS_ERR_INVALID, "unknown precision %d", prec);
   if (prec == KNNS_PREC_FP64 && c->dtype != KNNS_F64) return fail(KNNS_ERR_INVALID, "FP64 precision needs a double corpus");
   if (prec == KNNS_PREC_BF16 && !c->x16) return fail(KNNS_ERR_INVALID, "BF16 precision needs a corpus built with KNNS_CORPUS_BF16");
+  if (prec == KNNS_PREC_FP8 && !c->x8) return fail(KNNS_ERR_INVALID, "FP8 precision needs a corpus built with KNNS_CORPUS_FP8");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     (void)hipGetLastError();
@@ -254,10 +254,10 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
     const knns::SearchSizes y = knns::search_sizes(c, P, cap, cap * (long)cfg->max_k, false, S->osz == 8);
     knns::SearchSizes t = z;
 #define SRV_MAX(f) t.f = std::max(t.f, y.f)
-    SRV_MAX(q32); SRV_MAX(q16); SRV_MAX(q64); SRV_MAX(thr); SRV_MAX(kc); SRV_MAX(kq); SRV_MAX(off);
+    SRV_MAX(q32); SRV_MAX(q16); SRV_MAX(q8); SRV_MAX(q64); SRV_MAX(thr); SRV_MAX(kc); SRV_MAX(kq); SRV_MAX(off);
     SRV_MAX(sub); SRV_MAX(subcnt); SRV_MAX(carry); SRV_MAX(ccnt); SRV_MAX(flags); SRV_MAX(odist); SRV_MAX(oidx);
 #undef SRV_MAX
-    const long bytes = (long)(t.q32 + t.q16 + t.q64 + t.thr + t.kc + t.kq + t.off + t.sub + t.subcnt +
+    const long bytes = (long)(t.q32 + t.q16 + t.q8 + t.q64 + t.thr + t.kc + t.kq + t.off + t.sub + t.subcnt +
                               2 * t.carry + 2 * t.ccnt + t.flags + t.odist + t.oidx) +
                        cap * (long)c->d * (long)S->esz + 256;
     if (budget > 0 && bytes > budget) break;   // admission by memory: larger buckets do not fit
@@ -285,7 +285,7 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
     return true;
   };
   knns::SearchBuffers& B = S->B;
-  bool ok = dalloc(z.q32, &B.q32) && dalloc(z.q16, &B.q16) && dalloc(z.q64, &B.q64) && dalloc(z.thr, &B.thr) &&
+  bool ok = dalloc(z.q32, &B.q32) && dalloc(z.q16, &B.q16) && dalloc(z.q8, &B.q8) && dalloc(z.q64, &B.q64) && dalloc(z.thr, &B.thr) &&
             dalloc(z.kc, &B.kc) && dalloc(z.kq, &B.kq) && dalloc(z.off, &B.off) && dalloc(z.sub, &B.sub) &&
             dalloc(z.subcnt, &B.subcnt) && dalloc(z.carry, &B.carry[0]) && dalloc(z.carry, &B.carry[1]) &&
             dalloc(z.ccnt, &B.ccnt[0]) && dalloc(z.ccnt, &B.ccnt[1]) && dalloc(z.flags, &B.flags) &&

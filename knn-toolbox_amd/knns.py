@@ -184,7 +184,7 @@ class Corpus:
     """A corpus resident in device memory (native API)."""
 
     def __init__(self, data, bf16: bool = False, device_ptr: bool = False, m: int | None = None,
-                 d: int | None = None, dtype=None):
+                 d: int | None = None, dtype=None, fp8: bool = False):
         L = native()
         if device_ptr:
             self.m, self.d = int(m), int(d)
@@ -197,7 +197,8 @@ class Corpus:
             self.m, self.d = data.shape
             self.dtype = data.dtype
             src = data.ctypes.data
-        flags = (KNNS_CORPUS_BF16 if bf16 else 0) | (KNNS_CORPUS_DEVICE_PTR if device_ptr else 0)
+        flags = ((KNNS_CORPUS_BF16 if bf16 else 0) | (KNNS_CORPUS_FP8 if fp8 else 0) |
+                 (KNNS_CORPUS_DEVICE_PTR if device_ptr else 0))
         h = ctypes.c_void_p()
         st = L.knns_corpus_create(ctypes.byref(h), src, KNNS_F64 if self.dtype == np.float64 else KNNS_F32,
                                   self.m, self.d, flags)

@@ -17,6 +17,8 @@ Contents
     - ``ref_tbis_select``   src/cuknnBitonic.cu:28-254 (Comparator, Comparator_elim, bitonic_shared,
                             bitonic_shared2, relloc, BitonicSelect)
     - ``ref_merge_res``     src/cuknnBitonic.cu:381-417 (mergeResBitonic host merge)
+* The FP8 shortlist model (``e4m3_encode``, ``fp8_rows``, ``fp8_keys``): OCP E4M3 with a per-row
+  power-of-two scale, the quantiser of the precision form (no counterpart in the reference).
   They pin what the reference *selects*: the k smallest keys. tests/test_oracle.py checks them
   against a stable sort on tie-free keys and records their tie behaviour.
 """
@@ -390,6 +392,75 @@ def ref_merge_res(data: np.ndarray, idx: np.ndarray, k: int, Q: int, P: int):
                             cmax = data[s * k + q]
                             maxid = q
     return data[: Q * k].reshape(Q, k), idx[: Q * k].reshape(Q, k)
+
+
+# ----------------------------------------------------------------------------------------------
+# FP8 shortlist model (the precision form's quantiser; it has no counterpart in the reference,
+# whose path is FP32/FP64 only). Pinned by tests/test_fp8.py against torch.float8_e4m3fn, the
+# E4M3 code table and brute force over the exponent range.
+# ----------------------------------------------------------------------------------------------
+E4M3_MAX = 448.0
+
+
+def e4m3_encode(x) -> np.ndarray:
+    """OCP E4M3 codes (1 sign, 4 exponent bits with bias 7, 3 mantissa bits; no infinities,
+    0x7F/0xFF are NaN): round to nearest, ties to even, saturating at +-448."""
+    x = np.asarray(x, dtype=np.float64)
+    a = np.abs(x)
+    code = np.zeros(a.shape, dtype=np.int64)
+    sub = a < 2.0 ** -6                      # subnormals k * 2^-9, k = 0..8 (8 = smallest normal)
+    code[sub] = np.rint(a[sub] * 512.0).astype(np.int64)
+    nrm = ~sub
+    f, e = np.frexp(a[nrm])                  # a = f * 2^e, f in [0.5, 1)
+    e = e - 1                                # a = (2f) * 2^e
+    q = np.rint((2.0 * f - 1.0) * 8.0).astype(np.int64)
+    carry = q == 8
+    e = e + carry
+    q = np.where(carry, 0, q)
+    sat = (e > 8) | ((e == 8) & (q == 7))
+    code[nrm] = np.where(sat, 0x7E, ((e + 7) << 3) | q)
+    code[a >= E4M3_MAX] = 0x7E
+    return (code | np.where(np.signbit(x), 0x80, 0)).astype(np.uint8)
+
+
+def e4m3_decode(b) -> np.ndarray:
+    b = np.asarray(b, dtype=np.int64)
+    ex, mnt = (b >> 3) & 15, b & 7
+    v = np.where(ex == 0, mnt * 2.0 ** -9, (1.0 + mnt / 8.0) * np.ldexp(1.0, ex - 7))
+    return np.where(b & 0x80, -v, v)
+
+
+def fp8_scale_exp(amax) -> np.ndarray:
+    """Per-row power-of-two scale: the smallest p with amax / 2^p <= 448 (0 for amax = 0), clamped
+    to [-126, 126] (the E8M0 scale range used)."""
+    amax = np.asarray(amax, dtype=np.float64)
+    p = np.zeros(amax.shape, dtype=np.int64)
+    pos = amax > 0
+    f, e = np.frexp(amax[pos] / E4M3_MAX)
+    pp = np.where(f == 0.5, e - 1, e)        # ceil(log2(amax / 448)) of the rounded quotient
+    for _ in range(2):                       # exact correction of the quotient's rounding
+        pp = np.where(amax[pos] / np.ldexp(1.0, pp) > E4M3_MAX, pp + 1, pp)
+        pp = np.where((pp > -126) & (amax[pos] / np.ldexp(1.0, pp - 1) <= E4M3_MAX), pp - 1, pp)
+    p[pos] = pp
+    return np.clip(p, -126, 126)
+
+
+def fp8_rows(X: np.ndarray):
+    """Quantise rows as the FP8 operand copy does: FP32 values, per-row scale 2^p, E4M3 codes.
+    Returns (codes uint8 (n, d), p int (n,), dequantised rows r~ = 2^p * e4m3(r / 2^p) float64)."""
+    Xf = np.asarray(X).astype(np.float32).astype(np.float64)
+    amax = np.abs(Xf).max(axis=1) if Xf.shape[1] else np.zeros(Xf.shape[0])
+    p = fp8_scale_exp(amax)
+    codes = e4m3_encode(Xf * np.ldexp(1.0, -p)[:, None])
+    return codes, p, e4m3_decode(codes) * np.ldexp(1.0, p)[:, None]
+
+
+def fp8_keys(Q: np.ndarray, R: np.ndarray) -> np.ndarray:
+    """The FP8 shortlist's ranking key for every pair: ||q~ - r~||^2 of the dequantised rows
+    (the scan ranks by -(q~.r~ - ||r~||^2 / 2), which orders pairs of one query alike)."""
+    _, _, Qd = fp8_rows(Q)
+    _, _, Rd = fp8_rows(R)
+    return (Qd * Qd).sum(1)[:, None] - 2.0 * (Qd @ Rd.T) + (Rd * Rd).sum(1)[None, :]
 
 
 # ----------------------------------------------------------------------------------------------

@@ -23,7 +23,7 @@ def test_server_config_rejected_before_device_work(knns_mod):
     for prec, max_k, max_batch, budget, want in [
             (0, 0, 64, 0, knns_mod.KNNS_ERR_INVALID), (0, 1025, 64, 0, knns_mod.KNNS_ERR_INVALID),
             (0, 10, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 10, 64, -1, knns_mod.KNNS_ERR_INVALID),
-            (2, 10, 64, 0, knns_mod.KNNS_ERR_UNSUPPORTED), (7, 10, 64, 0, knns_mod.KNNS_ERR_INVALID)]:
+            (7, 10, 64, 0, knns_mod.KNNS_ERR_INVALID)]:
         cfg = knns_mod.ServerConfig(prec, max_k, max_batch, budget)
         assert L.knns_server_create(ctypes.byref(h), fake, ctypes.byref(cfg)) == want
         assert not h.value
@@ -64,12 +64,12 @@ def _requests(Q, count, seed, kset=(1, 10, 100), nmax=200):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("precision", ["bf16", "fp32", "fp8"])
 def test_serving_mix_equals_alone_equals_eager(gpu, knns_mod, mixture_corpus, precision):
     R, Q = mixture_corpus
     reqs = _requests(Q, 40, seed=7)
     print(f"\n[case] serving precision={precision} m={R.shape[0]} requests={len(reqs)}", flush=True)
-    with knns_mod.Corpus(R, bf16=precision == "bf16") as corpus:
+    with knns_mod.Corpus(R, bf16=precision == "bf16", fp8=precision == "fp8") as corpus:
         eager = [corpus.search(q, k, precision=precision) for q, k in reqs]
         with knns_mod.Server(corpus, precision=precision, max_k=100, max_batch=1024) as srv:
             # 1. all requests in flight together (batched into mixes)
