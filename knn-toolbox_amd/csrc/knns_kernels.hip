@@ -459,29 +459,24 @@ __device__ __forceinline__ void scan_epilogue(const ScanArgs& a, ScanState<QB>& 
     const float phi = st.phi[qb];
     if (__builtin_amdgcn_ballot_w64(qm > phi) == 0) continue;
     uint64_t* pool = a.sub + st.pbase(a, qb);
+    int cnt = st.cnt[qb];
 #pragma unroll
     for (int gi = 0; gi < 8; ++gi) {
       if (__builtin_amdgcn_ballot_w64(gm[gi] > phi) == 0) continue;
-      if (gm[gi] > phi) {
-        // lanes holding a survivor in this group of 4 rows append each survivor (usually one):
-        // a loop over the set bits of the group's survivor mask, one store per survivor (room
-        // for a whole stage is guaranteed before the stage starts)
-        const int cb = gi >> 2, v0 = (gi & 3) * 4;
-        const float d0 = Dv[cb][v0], d1 = Dv[cb][v0 + 1], d2 = Dv[cb][v0 + 2], d3 = Dv[cb][v0 + 3];
-        uint32_t bits = (d0 > phi ? 1u : 0u) | (d1 > phi ? 2u : 0u) | (d2 > phi ? 4u : 0u) |
-                        (d3 > phi ? 8u : 0u);
-        const uint32_t rg = tb + (uint32_t)(cb * 32 + 8 * (gi & 3));
-        int cnt = st.cnt[qb];
-        do {
-          const uint32_t b = (uint32_t)__builtin_ctz(bits);
-          bits &= bits - 1;
-          const float dl = (b & 1u) ? d1 : d0, dh = (b & 1u) ? d3 : d2;
-          const float D = (b & 2u) ? dh : dl;
-          pool[cnt++] = ((uint64_t)__builtin_bit_cast(uint32_t, D) << 32) | (rg + b);
-        } while (bits);
-        st.cnt[qb] = cnt;
+      // some lane holds a survivor in this group of 4 rows: one compare per value, whose result
+      // is the exec mask of its store (room for a whole stage is guaranteed before the stage)
+      const int cb = gi >> 2, v0 = (gi & 3) * 4;
+      const uint32_t rg = tb + (uint32_t)(cb * 32 + 8 * (gi & 3));
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float D = Dv[cb][v0 + t];
+        if (D > phi) {
+          pool[cnt] = ((uint64_t)__builtin_bit_cast(uint32_t, D) << 32) | (rg + (uint32_t)t);
+          ++cnt;
+        }
       }
     }
+    st.cnt[qb] = cnt;
   }
 }
 
