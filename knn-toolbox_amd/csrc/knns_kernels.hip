@@ -816,20 +816,17 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
   const uint32_t dma_off = (uint32_t)(lane / SU) * ldb + (uint32_t)((lane % SU) ^ swl) * 16u;
   const uint32_t aug_off = (uint32_t)lane * ldb + (uint32_t)a.dp * ESZ;
   typedef __attribute__((address_space(3))) uint8_t lds_u8;
-  auto issue = [&](int t0, int buf) {
-    // running scalar source and LDS pointers (the empty asm keeps them in SGPRs and stops the
-    // compiler from folding the lane offset into per-load 64-bit vector addresses)
-    const uint8_t* p = X + (size_t)(t0 + wv * G::RPL) * ldb;
-    lds_u8* l = (lds_u8*)(smem + buf * G::MAIN_BYTES + wv * 1024);
-#pragma unroll
-    for (int k = 0; k < G::PER_WAVE; ++k) {
+  // Load k of a stage (k < PER_WAVE: main units of rows wv*RPL + 4*RPL*k; k == PER_WAVE: the
+  // augmentation units of rows 64*wv, waves 0 and 1 only). Scalar source and LDS addresses (the
+  // empty asm keeps them in SGPRs instead of per-load 64-bit vector addresses).
+  auto issue_one = [&](int t0, int buf, int k) {
+    if (k < G::PER_WAVE) {
+      const uint8_t* p = X + (size_t)(t0 + wv * G::RPL + k * 4 * G::RPL) * ldb;
+      lds_u8* l = (lds_u8*)(smem + buf * G::MAIN_BYTES + wv * 1024 + k * 4096);
       asm volatile("" : "+s"(p), "+s"(l));
       __builtin_amdgcn_global_load_lds((const void*)(p + dma_off), (void __attribute__((address_space(3)))*)l,
                                        16, 0, 0);
-      p += (size_t)(4 * G::RPL) * ldb;
-      l += 4096;
-    }
-    if (wv < G::SR / 64) {
+    } else if (wv < G::SR / 64) {
       const uint8_t* pa = X + (size_t)(t0 + wv * 64) * ldb;
       lds_u8* la = (lds_u8*)(smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES + wv * 1024);
       asm volatile("" : "+s"(pa), "+s"(la));
@@ -837,6 +834,12 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
                                        16, 0, 0);
     }
   };
+  // The next stage's NL loads are spread over this stage's 2*J k-steps (a burst of them right
+  // after the barrier contends with the operand reads): step t issues loads
+  // [t*NL/(2J), (t+1)*NL/(2J)). A stage with one sub-tile is the slice's last, which prefetches
+  // nothing.
+  constexpr int NL = G::PER_WAVE + 1;
+  constexpr int STEPS = 2 * J;
 
   // ---- per-wave query operand (B), resident for the whole slice ----
   typedef typename std::conditional<F8, i32x8, bf16x8>::type OT;
@@ -871,14 +874,17 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
   const uint32_t A0 = (uint32_t)col * SU * 16u + (uint32_t)(((h * (SU / 2)) ^ swc) * 16);
   const uint32_t AG = (uint32_t)col * 16u;
 
-  if (nst > 0) issue(c0, 0);
+  if (nst > 0) {
+#pragma unroll
+    for (int k = 0; k < NL; ++k) issue_one(c0, 0, k);
+  }
   f32x16 acc[QB][2];
   for (int s = 0; s < nst; ++s) {
     __builtin_amdgcn_s_waitcnt(0);   // this wave's DMA for stage s has landed
     __syncthreads();                 // ... every wave's; and nobody still reads the other buffer
     const int buf = s & 1;
     const int t0s = c0 + s * G::SR;
-    if (s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
+    const bool pf = s + 1 < nst;
     const uint8_t* mb = smem + buf * G::MAIN_BYTES;
     const uint8_t* ab = smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES;
 #pragma unroll
@@ -929,6 +935,11 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
             acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, bq[qb][j], acc[qb][0], 0, 0, 0);
             acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, bq[qb][j], acc[qb][1], 0, 0, 0);
           }
+        }
+        if (pf) {
+          const int step = sub * J + j;
+#pragma unroll
+          for (int k = step * NL / STEPS; k < (step + 1) * NL / STEPS; ++k) issue_one(t0s + G::SR, buf ^ 1, k);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
