@@ -261,7 +261,7 @@ def main() -> int:
     R, Q = oracle.workload_c2(m=m, n=n)
     log(f"generated corpus {R.shape} queries {Q.shape} in {time.time() - t0:.1f}s")
 
-    corpus = knns.Corpus(R, bf16=True, fp8=True)
+    corpus = knns.Corpus(R, bf16=True)
     dev = torch.device("cuda:0")
     Qd = torch.from_numpy(Q).to(dev)
     Dd = torch.empty((n, k), dtype=torch.float32, device=dev)

@@ -1562,7 +1562,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
 #undef KNNS_F8_CASE
     return hipErrorInvalidValue;
   }
-  if (prec == PREC_BF16 && a.dp == dc && dc != 96 && !std::getenv("KNNS_SCAN_V2")) {
+  if (prec == PREC_BF16 && a.dp == dc && dc != 96) {
 #define KNNS_SWZ_CASE(B, D) \
     if (qb == B && dc == D) return scan_grid(scan_swz_kernel<B, D / 8, false>, a, groups, stream);
     KNNS_SWZ_CASE(1, 32) KNNS_SWZ_CASE(1, 64) KNNS_SWZ_CASE(1, 128)
@@ -1585,7 +1585,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
 hipError_t launch_merge(const MergeArgs& a, hipStream_t stream) {
   if (a.n <= 0) return hipSuccess;
   if (1 + 2 * a.slices > MERGE_MAX_SRC) return hipErrorInvalidValue;
-  if (a.bitonic || std::getenv("KNNS_MERGE_V1"))
+  if (a.bitonic)
     hipLaunchKernelGGL(merge_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   else
     hipLaunchKernelGGL(merge_reg_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
