@@ -834,12 +834,12 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
                                        16, 0, 0);
     }
   };
-  // The next stage's NL loads are spread over this stage's 2*J k-steps (a burst of them right
-  // after the barrier contends with the operand reads): step t issues loads
-  // [t*NL/(2J), (t+1)*NL/(2J)). A stage with one sub-tile is the slice's last, which prefetches
-  // nothing.
-  constexpr int NL = G::PER_WAVE + 1;
-  constexpr int STEPS = 2 * J;
+  // all loads of a stage, issued right after the barrier (spreading them over the k-steps was
+  // measured: no gain for BF16, 4% slower for FP8)
+  auto issue = [&](int t0, int buf) {
+#pragma unroll
+    for (int k = 0; k <= G::PER_WAVE; ++k) issue_one(t0, buf, k);
+  };
 
   // ---- per-wave query operand (B), resident for the whole slice ----
   typedef typename std::conditional<F8, i32x8, bf16x8>::type OT;
@@ -874,17 +874,14 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
   const uint32_t A0 = (uint32_t)col * SU * 16u + (uint32_t)(((h * (SU / 2)) ^ swc) * 16);
   const uint32_t AG = (uint32_t)col * 16u;
 
-  if (nst > 0) {
-#pragma unroll
-    for (int k = 0; k < NL; ++k) issue_one(c0, 0, k);
-  }
+  if (nst > 0) issue(c0, 0);
   f32x16 acc[QB][2];
   for (int s = 0; s < nst; ++s) {
     __builtin_amdgcn_s_waitcnt(0);   // this wave's DMA for stage s has landed
     __syncthreads();                 // ... every wave's; and nobody still reads the other buffer
     const int buf = s & 1;
     const int t0s = c0 + s * G::SR;
-    const bool pf = s + 1 < nst;
+    if (s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
     const uint8_t* mb = smem + buf * G::MAIN_BYTES;
     const uint8_t* ab = smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES;
 #pragma unroll
@@ -935,11 +932,6 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
             acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, bq[qb][j], acc[qb][0], 0, 0, 0);
             acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, bq[qb][j], acc[qb][1], 0, 0, 0);
           }
-        }
-        if (pf) {
-          const int step = sub * J + j;
-#pragma unroll
-          for (int k = step * NL / STEPS; k < (step + 1) * NL / STEPS; ++k) issue_one(t0s + G::SR, buf ^ 1, k);
         }
         __builtin_amdgcn_sched_barrier(0);
       }

@@ -193,6 +193,14 @@ def workload_c2(m: int = 1_000_000, n: int = 10_000):
     return R, Q
 
 
+def workload_c5(m: int = 20_000_000, n: int = 20_000):
+    """Serving corpus: 16384-centre mixture, sigma 0.1, d=128: corpus seed 8; the query pool that
+    requests draw from uses the same centres (seed 10; seed 9 drives the arrival process)."""
+    R = mixture(m, 128, 16384, 0.1, center_seed=8, point_seed=8 + 1000)
+    Q = mixture(n, 128, 16384, 0.1, center_seed=8, point_seed=10 + 1000)
+    return R, Q
+
+
 def workload_c3(m: int = 10_000_000, n: int = 10_000):
     R = mixture(m, 96, 4096, 0.15, center_seed=4, point_seed=4 + 1000)
     Q = mixture(n, 96, 4096, 0.15, center_seed=4, point_seed=5 + 1000)
