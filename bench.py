@@ -147,15 +147,20 @@ def run_c5(args, knns, oracle, torch) -> dict:
     log(f"C5 corpus resident in {time.time() - t0:.1f}s")
     m, d = R.shape
     del R
-    srv = run_serving(knns, corpus, Q, args.serving_seconds, fractions=(0.25, 0.5, 0.75, 0.9),
-                      corpus_desc=f"{m:,} x {d} BF16 (resident, FP32 master), shortlist k+k/2+16, exact FP32 rerank")
+    desc = f"{m:,} x {d} BF16 (resident, FP32 master), shortlist k+k/2+16, exact FP32 rerank"
+    # the batch cap trades throughput for tail latency: a batch of 16384 queries on this corpus
+    # runs ~90 ms, and requests that arrive meanwhile wait for it
+    runs = {mb: run_serving(knns, corpus, Q, args.serving_seconds, fractions=(0.25, 0.5, 0.75, 0.9),
+                            max_batch=mb, corpus_desc=desc) for mb in (16384, 4096)}
     corpus.close()
+    srv = runs[4096]           # the better of the two caps on this corpus (throughput and latency)
     return {"metric": "serving queries/s and p50/p99 latency (corpus 20M x 128, continuous batching)",
             "value": srv["saturation_qps"], "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
             "dtype": "bf16", "data": "synthetic (16384-centre Gaussian mixture, sigma 0.1, numpy PCG64 seeds 8/10; arrivals seed 9)",
             "config": {"workload": "C5: corpus 20,000,000 x 128 resident, Poisson requests of 1..4096 queries, k in {1,10,100}",
-                       "m": m, "d": d, "max_batch": 16384, "seconds_per_phase": args.serving_seconds},
-            "p50_ms": srv["p50_ms"], "p99_ms": srv["p99_ms"], "serving": srv}
+                       "m": m, "d": d, "max_batch": 4096, "seconds_per_phase": args.serving_seconds},
+            "p50_ms": srv["p50_ms"], "p99_ms": srv["p99_ms"], "serving": srv,
+            "serving_max_batch_16384": runs[16384]}
 
 
 def timed(run, steps: int, warmup: int = 1):

@@ -395,6 +395,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     sa.flags = (int*)B.flags;
     sa.wg_rows = P.groups / 4;
     sa.dense = R.dense;
+    sa.n_dev = L.n_dev;
     sa.xcd_swizzle = 0;   // XCD-aware slice mapping: measured, no gain on 1M x 128 (kept off)
     if (hook) (*hook)(0);
     KNNS_TRY(launch_scan(P.prec_kernel, P.qb, P.dc, sa, P.groups, st));

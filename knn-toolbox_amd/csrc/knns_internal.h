@@ -51,6 +51,8 @@ struct ScanArgs {
   int wg_rows;           // query workgroup rows (groups / 4) when xcd_swizzle
   int dense;             // 1: no filter; every row is written at its slice-relative position of the
                          // (query, slice, half 0) sub-pool (slice_rows <= sub_cap), fills = rows
+  const int* n_dev;      // optional device query count: workgroups whose queries all lie past it
+                         // return at once (graph-captured searches over a padded slot count)
 };
 
 struct MergeArgs {

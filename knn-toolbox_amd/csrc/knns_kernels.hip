@@ -562,6 +562,8 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
     wgrow = blockIdx.x;
     sl = blockIdx.y;
   }
+  // every query of the workgroup is an inert slot of a padded (graph-captured) search
+  if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
   const int g = wgrow * 4 + wv;   // query group of this wave
   const long c0 = a.row_begin + (long)sl * a.slice_rows;
   long c1 = c0 + a.slice_rows;
@@ -792,6 +794,8 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
     wgrow = blockIdx.x;
     sl = blockIdx.y;
   }
+  // every query of the workgroup is an inert slot of a padded (graph-captured) search
+  if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
   const int g = wgrow * 4 + wv;
   // corpus rows are int32 indices: every row offset below fits 32 bits
   const int c0 = (int)(a.row_begin + (long)sl * a.slice_rows);
