@@ -201,3 +201,30 @@ def test_generators_deterministic_and_shaped(oracle_mod):
     assert abs(d2.mean() - 16 * 0.01) < 0.01
     s = oracle_mod.log_uniform_sizes(64, 64, 16384, 7, total=200_000)
     assert s.sum() == 200_000 and len(s) == 64 and s.min() >= 1
+
+
+def test_exact_topk_and_knnm_property_tiny_integer_inputs(oracle_mod):
+    """Random tiny shapes with many exact ties (hypothesis): the FP64 brute force equals the pure
+    Python integer brute force, and the knn.m transcription (exact on small integers) agrees."""
+    hypothesis = pytest.importorskip("hypothesis")
+    from hypothesis import given, settings, strategies as hst
+
+    @settings(max_examples=60, deadline=None, derandomize=True)
+    @given(m=hst.integers(1, 40), n=hst.integers(1, 6), d=hst.integers(1, 6), k=hst.integers(1, 45),
+           hi=hst.integers(1, 3), seed=hst.integers(0, 2 ** 16), excl=hst.booleans())
+    def check(m, n, d, k, hi, seed, excl):
+        rng = np.random.default_rng(seed)
+        R = rng.integers(0, hi + 1, (m, d)).astype(np.float32)
+        Q = R[:n].copy() if excl and n <= m else rng.integers(0, hi + 1, (n, d)).astype(np.float32)
+        self_ids = np.arange(n) if excl and n <= m else None
+        kk = min(k, m - (1 if self_ids is not None else 0))
+        if kk < 1:
+            return
+        D, I = oracle_mod.exact_topk(Q, R, kk, self_ids=self_ids)
+        Dp, Ip = _py_exact(Q, R, kk, self_ids=self_ids)
+        assert np.array_equal(I, Ip) and np.array_equal(D, Dp)
+        if self_ids is None:
+            Dm, Im = oracle_mod.knnm(Q, R, kk)
+            assert np.array_equal(Im, Ip) and np.array_equal(Dm.astype(np.float64), Dp)
+
+    check()
