@@ -2,7 +2,8 @@
 knobs that give the fastest search, and writes them as the tuning table the engine reads
 (knn-toolbox_amd/tuning/gfx950.tsv; see knns_set_tuning_table in include/knns_native.h).
 
-    python knn-toolbox_amd/tune.py [--m 1000000] [--quick] [--out PATH]
+    python knn-toolbox_amd/tune.py [--m 1000000] [--quick] [--precisions bf16,fp32,fp8]
+                                   [--merge] [--out PATH]
 
 Knobs: min_waves (slices per query group: the smallest count that fills that many waves of
 resident scan workgroups) and ratio (corpus growth between threshold rounds; 0 = x4 while the
@@ -80,12 +81,25 @@ def main() -> int:
     ap.add_argument("--m", type=int, default=1_000_000)
     ap.add_argument("--reps", type=int, default=9)
     ap.add_argument("--quick", action="store_true", help="one point (bf16, d=128, k=100, n=10000)")
+    ap.add_argument("--precisions", default="bf16,fp32", help="comma-separated: bf16, fp32, fp8")
+    ap.add_argument("--merge", action="store_true",
+                    help="keep the entries of the other precisions from the table in the package")
     ap.add_argument("--out", default=str(PKG / "tuning" / "gfx950.tsv"))
     args = ap.parse_args()
     if knns.device_count() <= 0:
         print("no GPU visible", file=sys.stderr)
         return 3
-    precs = ("bf16",) if args.quick else ("bf16", "fp32")
+    precs = ("bf16",) if args.quick else tuple(p.strip() for p in args.precisions.split(",") if p.strip())
+    for p in precs:
+        if p not in ("bf16", "fp32", "fp8"):
+            raise SystemExit(f"unknown precision {p}")
+    kept = []
+    base = PKG / "tuning" / "gfx950.tsv"
+    if args.merge and base.exists():
+        for line in base.read_text().splitlines():
+            f = line.split()
+            if f and not line.startswith("#") and f[0] not in precs:
+                kept.append(line)
     dims = (128,) if args.quick else (32, 64, 128, 256)
     ks = (100,) if args.quick else (10, 100)
     ns = (10000,) if args.quick else (256, 2048, 10000)
@@ -96,7 +110,7 @@ def main() -> int:
         R = mixture(args.m, d, 1, centers)
         Qall = mixture(max(ns), d, 2, centers)
         for prec in precs:
-            with knns.Corpus(R, bf16=prec == "bf16") as corpus:
+            with knns.Corpus(R, bf16=prec == "bf16", fp8=prec == "fp8") as corpus:
                 for k in ks:
                     for n in ns:
                         Q = Qall[:n]
@@ -123,9 +137,11 @@ def main() -> int:
                 f"# {datetime.date.today().isoformat()}, corpus m={args.m} (1024-centre Gaussian mixture), "
                 f"median of {args.reps} searches\n"
                 f"# precision d k n min_waves ratio ms\n")
+        for line in kept:
+            f.write(line + "\n")
         for r in rows:
             f.write(f"{r[0]} {r[1]} {r[2]} {r[3]} {r[4]} {r[5]} {r[6]:.3f}\n")
-    print(f"wrote {out} ({len(rows)} entries)")
+    print(f"wrote {out} ({len(rows)} new entries, {len(kept)} kept)")
     return 0
 
 
