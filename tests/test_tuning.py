@@ -18,8 +18,13 @@ def test_committed_table_parses(knns_mod, restore_table):
     L = knns_mod.native()
     assert L.knns_set_tuning_table(str(TABLE).encode()) == 0
     rows = [l.split() for l in TABLE.read_text().splitlines() if l.strip() and not l.startswith("#")]
-    assert len(rows) >= 40
-    assert {r[0] for r in rows} == {"bf16", "fp32"}
+    assert len(rows) >= 60
+    assert {r[0] for r in rows} == {"bf16", "fp32", "fp8"}
+    # every entry: d, k, n and the two knobs are integers in range, the time is positive
+    for r in rows:
+        d, k, n, w, ratio = map(int, r[1:6])
+        assert d > 0 and 1 <= k <= 1024 and n > 0 and w >= 1 and (ratio == 0 or ratio >= 2)
+        assert float(r[6]) > 0
 
 
 def test_table_drives_slices_and_ratio(knns_mod, tmp_path, restore_table):
