@@ -182,3 +182,13 @@ def test_fp8_needs_fp8_corpus(gpu, knns_mod):
     with knns_mod.Corpus(R) as c:
         with pytest.raises(knns_mod.KnnsError):
             c.search(R[:4], 3, precision="fp8")
+
+
+def test_fp8_plan_uses_fp8_row_bytes(knns_mod):
+    # the FP8 scan's chunk is the FP8 row (d rounded up to a whole 64-byte MFMA step); the FP32
+    # rerank rows keep their own padding
+    for d, dc, dp in [(1, 64, 32), (24, 64, 32), (64, 64, 64), (65, 128, 96), (96, 128, 96), (128, 128, 128),
+                      (200, 256, 256), (256, 256, 256)]:
+        p = knns_mod.plan_describe(100_000, d, 1000, 10, "fp8")
+        assert (p["dc"], p["dp"], p["qb"]) == (dc, dp, 2), d
+    assert knns_mod.plan_describe(100_000, 96, 20, 10, "fp8")["qb"] == 1
