@@ -47,7 +47,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_serving(knns, corpus, Q, seconds: float, fractions=(0.5,), max_batch: int = 16384,
+def run_serving(knns, corpus, Q, seconds: float, fractions=(0.5,), max_batch: int = 4096,
                 corpus_desc: str = "1,000,000 x 128 BF16 (resident), shortlist k+k/2+16, exact FP32 rerank") -> dict:
     """Requests of log-uniform size in [1, 4096] queries (drawn from Q) with k uniform in
     {1, 10, 100}. Phase 1, closed loop: 8 requests kept in flight -> saturation q/s. Phase 2,
@@ -268,6 +268,8 @@ def main() -> int:
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-host-io", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--serving-max-batch", type=int, default=4096,
+                    help="batch cap (largest bucket) of the headline serving phase")
     ap.add_argument("--serving-seconds", type=float, default=3.0,
                     help="duration of each serving phase (saturation, then Poisson at half load)")
     ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
@@ -385,7 +387,7 @@ def main() -> int:
     # ---- serving: continuous batching over the resident corpus (knns_server_*) ----
     serving = None
     if not args.skip_serving:
-        serving = run_serving(knns, corpus, Q, args.serving_seconds)
+        serving = run_serving(knns, corpus, Q, args.serving_seconds, max_batch=args.serving_max_batch)
         log(f"serving: {serving}")
 
     # ---- recall vs the FP64 oracle on a query sample ----
