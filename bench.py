@@ -369,6 +369,25 @@ def main() -> int:
         fp32["identical_to_bf16"] = bool(np.array_equal(I_fp32, I_bf16))
         log(f"fp32: {fp32}")
 
+    # ---- the workload's other variants: both entry points (threshold = heap, bitonic) and
+    # k in {10, 100}, each checked against the k=100 threshold answer (its first k columns) ----
+    variants = None
+    if not args.skip_fp32 and k == 100:
+        variants = {}
+        for prec in ("fp32", "bf16"):
+            for sel in ("threshold", "bitonic"):
+                for kv in (100, 10):
+                    Dv = torch.empty((n, kv), dtype=torch.float32, device=dev)
+                    Iv = torch.empty((n, kv), dtype=torch.int32, device=dev)
+
+                    def runv():
+                        corpus.search_device(Qd, n, kv, Dv, Iv, precision=prec, select=sel)
+                    sec = timed(runv, 2 if prec == "fp32" else 5)
+                    same = bool(np.array_equal(Iv.cpu().numpy(), I_bf16[:, :kv]))
+                    variants[f"{prec}_{sel}_k{kv}"] = {"qps": n / sec, "ms_per_step": sec * 1e3,
+                                                       "identical_to_k100_threshold": same}
+        log(f"variants: {variants}")
+
     # ---- host-memory boundary: host queries in, host results out (corpus resident) and the whole
     # drop-in call (gpuknnsHeap: FP32, corpus uploaded and packed inside every call) ----
     host_io = None
@@ -447,6 +466,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "fp32_path": fp32,
         "speedup_vs_fp32": (qps / fp32["qps"]) if fp32 else None,
+        "variants": variants,
         "host_boundary": host_io,
         "p50_ms": serving["p50_ms"] if serving else None,
         "p99_ms": serving["p99_ms"] if serving else None,
