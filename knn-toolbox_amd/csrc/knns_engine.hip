@@ -393,10 +393,8 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     sa.sub_cap = P.sub_cap;
     sa.self_ids = (const int*)B.self;
     sa.flags = (int*)B.flags;
-    sa.wg_rows = P.groups / 4;
     sa.dense = R.dense;
     sa.n_dev = L.n_dev;
-    sa.xcd_swizzle = 0;   // XCD-aware slice mapping: measured, no gain on 1M x 128 (kept off)
     if (hook) (*hook)(0);
     KNNS_TRY(launch_scan(P.prec_kernel, P.qb, P.dc, sa, P.groups, st));
     if (hook) (*hook)(1);

@@ -47,8 +47,6 @@ struct ScanArgs {
   int sub_cap;
   const int* self_ids;   // optional: corpus row excluded for each query (all-kNN), -1 = none
   int* flags;            // [0] = overflow counter, [1] = slow-path compactions
-  int xcd_swizzle;       // 1: 1-D grid, slice sl runs on XCD sl % 8 (see scan_kernel)
-  int wg_rows;           // query workgroup rows (groups / 4) when xcd_swizzle
   int dense;             // 1: no filter; every row is written at its slice-relative position of the
                          // (query, slice, half 0) sub-pool (slice_rows <= sub_cap), fills = rows
   const int* n_dev;      // optional device query count: workgroups whose queries all lie past it

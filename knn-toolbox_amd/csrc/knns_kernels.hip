@@ -548,20 +548,8 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int col = lane & 31, h = lane >> 5;
-  // Workgroups are dispatched round-robin over the 8 XCDs (linear id % 8). With xcd_swizzle the
-  // linear id is mapped so that every workgroup of slice sl runs on XCD sl % 8: the query rows of a
-  // slice then share that XCD's L2 instead of every XCD fetching every slice.
-  int wgrow, sl;
-  if (a.xcd_swizzle) {
-    const int id = blockIdx.x;
-    const int xcd = id & 7, local = id >> 3;
-    wgrow = local % a.wg_rows;
-    sl = xcd + 8 * (local / a.wg_rows);
-    if (sl >= a.slices) return;   // padding workgroup (slices rounded up to a multiple of 8)
-  } else {
-    wgrow = blockIdx.x;
-    sl = blockIdx.y;
-  }
+  // grid: x = workgroup row (4 query groups), y = slice
+  const int wgrow = blockIdx.x, sl = blockIdx.y;
   // every query of the workgroup is an inert slot of a padded (graph-captured) search
   if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
   const int g = wgrow * 4 + wv;   // query group of this wave
@@ -783,17 +771,7 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar addressing
   const int col = lane & 31, h = lane >> 5;
-  int wgrow, sl;
-  if (a.xcd_swizzle) {
-    const int id = blockIdx.x;
-    const int xcd = id & 7, local = id >> 3;
-    wgrow = local % a.wg_rows;
-    sl = xcd + 8 * (local / a.wg_rows);
-    if (sl >= a.slices) return;
-  } else {
-    wgrow = blockIdx.x;
-    sl = blockIdx.y;
-  }
+  const int wgrow = blockIdx.x, sl = blockIdx.y;   // workgroup row (4 query groups), slice
   // every query of the workgroup is an inert slot of a padded (graph-captured) search
   if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
   const int g = wgrow * 4 + wv;
@@ -1517,13 +1495,7 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
 
 static hipError_t scan_grid(void (*kern)(ScanArgs), const ScanArgs& a, int groups, hipStream_t stream) {
   if (groups % 4) return hipErrorInvalidValue;  // a workgroup runs 4 query groups
-  if (a.xcd_swizzle) {
-    if (a.wg_rows != groups / 4) return hipErrorInvalidValue;
-    const long total = (long)a.wg_rows * ((a.slices + 7) / 8 * 8);
-    hipLaunchKernelGGL(kern, dim3((unsigned)total), dim3(256), 0, stream, a);
-  } else {
-    hipLaunchKernelGGL(kern, dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
-  }
+  hipLaunchKernelGGL(kern, dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
