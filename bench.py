@@ -279,7 +279,7 @@ def main() -> int:
                          "its FETCH_SIZE/WRITE_SIZE counters calibrate the HBM-traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
+        print(json.dumps({"error": "bench.py runs on one GPU; multi-GPU sharding is a design only "
                                    "(corpus-sharded top-k + one gather), so --gpus must be 1"}))
         return 2
 

@@ -21,7 +21,7 @@ typedef enum {
   KNNS_ERR_INVALID = 1,      /* bad argument (shape, k, dtype, precision/corpus mismatch) */
   KNNS_ERR_NOMEM = 2,        /* device or host allocation failed, or over the memory budget */
   KNNS_ERR_DEVICE = 3,       /* HIP runtime error or no device */
-  KNNS_ERR_UNSUPPORTED = 4,  /* valid request outside what this build implements */
+  KNNS_ERR_UNSUPPORTED = 4,  /* valid request outside what this library implements */
   KNNS_ERR_OVERFLOW = 5      /* candidate pool overflow (massive exact ties); results invalid */
 } knns_status;
 

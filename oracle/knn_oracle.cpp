@@ -1,7 +1,7 @@
 // CPU oracle for the brute-force kNN path. TEST INFRASTRUCTURE ONLY.
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library, and
-// only as the checker or as the timed CPU baseline. The GPU product path never calls it.
+// only to check results or as the timed CPU baseline. The GPU product path never calls it.
 //
 // Three restatements live here:
 //

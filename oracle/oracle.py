@@ -1,7 +1,7 @@
 """CPU oracle for the brute-force kNN path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import this
-module, and only as the checker (or the timed CPU baseline). The GPU product path never does.
+module, and only to check results (or as the timed CPU baseline). The GPU product path never does.
 
 Contents
 --------

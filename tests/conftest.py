@@ -10,7 +10,7 @@ for p in (ROOT / "oracle", ROOT / "knn-toolbox_amd", ROOT):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu on a machine that has one)")
     config.addinivalue_line("markers", "slow: longer CPU test")
 
 
@@ -33,5 +33,5 @@ def knns_mod():
 def gpu(knns_mod):
     n = knns_mod.device_count()
     if n <= 0:
-        pytest.fail("no GPU visible to the HIP runtime (GPU tests must run on the GPU box)")
+        pytest.fail("no GPU visible to the HIP runtime (GPU tests need an MI355X)")
     return n

@@ -4,7 +4,7 @@ Each demo (demo/*.m) draws random data, runs the GPU search and knn.m, and print
 er = norm(dist_gpu(:) - dist_knn_m(:)) / max|dist_knn_m| is below the script's tolerance; indices
 are never compared there. Here the same shapes and calls run through libcuknn / libcuknn_double
 (the reference's C interface, cuknns.h) on seeded U[0,1) data. Each test asserts the demo's own
-criterion and then this build's stricter parity: indices identical to the FP64 oracle (ties by
+criterion and then the library's stricter parity: indices identical to the FP64 oracle (ties by
 lower index) and distances equal to its value in the output type. The demo criterion is taken
 against the oracle's exact distances, not against knn.m's: knn.m evaluates ||r||^2 - 2 q.r +
 ||q||^2 in the input precision, whose cancellation error alone (about 1e-4 relative at d=128 in
