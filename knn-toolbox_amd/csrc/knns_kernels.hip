@@ -888,13 +888,12 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
       const u32x4 g0 = rd(ab + sub * 64 * 16, AG), g1 = rd(ab + sub * 64 * 16 + 32 * 16, AG);
       const int s0 = (int)(g0[2] & 0xFFu), s1 = (int)(g1[2] & 0xFFu);
       OT y0 = rdop(m0, A0), y1 = rdop(m1, A0);
-#pragma unroll
-      for (int qb = 0; qb < QB; ++qb) {
-        acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, g0), baug,
-                                                             (f32x16){0}, 0, 0, 0);
-        acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, g1), baug,
-                                                             (f32x16){0}, 0, 0, 0);
-      }
+      // the norm step does not depend on the queries: one MFMA per 32-row half into query block
+      // 0's accumulator, which every query block's first main step takes as its C operand
+      acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, g0), baug,
+                                                          (f32x16){0}, 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, g1), baug,
+                                                          (f32x16){0}, 0, 0, 0);
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const OT x0 = y0, x1 = y1;
@@ -902,17 +901,20 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
           const uint32_t o = A0 ^ (STEP * (uint32_t)(j + 1));
           y0 = rdop(m0, o); y1 = rdop(m1, o);
         }
+        // descending: at j = 0 the blocks qb > 0 read block 0's norm result before block 0 adds to it
 #pragma unroll
-        for (int qb = 0; qb < QB; ++qb) {
+        for (int qb = QB - 1; qb >= 0; --qb) {
+          const f32x16 c0 = j == 0 ? acc[0][0] : acc[qb][0];
+          const f32x16 c1 = j == 0 ? acc[0][1] : acc[qb][1];
           if constexpr (F8) {
             // formats 0, 0 = E4M3 x E4M3; scale bytes in bits 0..7 (op_sel 0)
-            acc[qb][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x0, bq[qb][j], acc[qb][0], 0, 0,
+            acc[qb][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x0, bq[qb][j], c0, 0, 0,
                                                                          0, s0, 0, sq[qb]);
-            acc[qb][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x1, bq[qb][j], acc[qb][1], 0, 0,
+            acc[qb][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x1, bq[qb][j], c1, 0, 0,
                                                                          0, s1, 0, sq[qb]);
           } else {
-            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, bq[qb][j], acc[qb][0], 0, 0, 0);
-            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, bq[qb][j], acc[qb][1], 0, 0, 0);
+            acc[qb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, bq[qb][j], c0, 0, 0, 0);
+            acc[qb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, bq[qb][j], c1, 0, 0, 0);
           }
         }
         __builtin_amdgcn_sched_barrier(0);
