@@ -21,7 +21,7 @@ struct knns_corpus_s {
   uint16_t* x16 = nullptr;
   double* x64 = nullptr;
   uint8_t* x8 = nullptr;   // FP8 (E4M3) rows with their scale, rows_pad x ld8 bytes
-  int dp8 = 0;             // FP8 row bytes (d rounded up to 64; at most 256)
+  int dp8 = 0;             // FP8 row bytes (d rounded up to 64, 128 or 256)
   long ld8 = 0;            // dp8 + 16 (the augmentation unit)
 };
 

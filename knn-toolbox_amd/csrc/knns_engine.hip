@@ -223,8 +223,9 @@ Tuning tuning_for(int prec, int d, int k, long n) {
   return best;
 }
 
-// FP8 operand row: d bytes rounded up to whole 64-byte MFMA steps
-int fp8_row_bytes(int d) { return (int)round_up(d, 64); }
+// FP8 operand row: d bytes rounded up to a row width the FP8 scan kernel is built for (64, 128 or
+// 256 bytes: its LDS swizzle needs a power-of-two count of 16-byte units); wider rows are rejected
+int fp8_row_bytes(int d) { return d <= 64 ? 64 : (d <= 128 ? 128 : (int)round_up(d, 256)); }
 
 // padded dimension and MFMA chunk for a dimension d
 void dims_for(int d, int* dp, int* dc) {

@@ -109,7 +109,7 @@ struct PackArgs {
   int is_query;          // aug = ones instead of -||r||^2/2
   uint8_t* x8;           // optional FP8 rows [rows_pad][ld8 bytes] (pack_fp8)
   long ld8;              // bytes per FP8 row: dp8 + 16
-  int dp8;               // FP8 padded dims (multiple of 64)
+  int dp8;               // FP8 row bytes (64, 128 or 256)
 };
 
 // Launchers (knns_kernels.hip). All enqueue on `stream` and return hipError_t.

@@ -185,10 +185,10 @@ def test_fp8_needs_fp8_corpus(gpu, knns_mod):
 
 
 def test_fp8_plan_uses_fp8_row_bytes(knns_mod):
-    # the FP8 scan's chunk is the FP8 row (d rounded up to a whole 64-byte MFMA step); the FP32
-    # rerank rows keep their own padding
+    # the FP8 scan's chunk is the FP8 row (d rounded up to 64, 128 or 256 bytes, the widths the
+    # kernel is built for); the FP32 rerank rows keep their own padding
     for d, dc, dp in [(1, 64, 32), (24, 64, 32), (64, 64, 64), (65, 128, 96), (96, 128, 96), (128, 128, 128),
-                      (200, 256, 256), (256, 256, 256)]:
+                      (129, 256, 256), (130, 256, 256), (200, 256, 256), (256, 256, 256)]:
         p = knns_mod.plan_describe(100_000, d, 1000, 10, "fp8")
         assert (p["dc"], p["dp"], p["qb"]) == (dc, dp, 2), d
     assert knns_mod.plan_describe(100_000, 96, 20, 10, "fp8")["qb"] == 1

@@ -162,3 +162,26 @@ def test_adversarial_order_forces_in_scan_compaction(gpu, knns_mod, oracle_mod, 
     assert st["slow_compactions"] > 0
     assert np.array_equal(ii.astype(np.int64), I.astype(np.int64))
     assert np.array_equal(dd, D.astype(np.float32))
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+@pytest.mark.parametrize("m,n,d,k", [
+    (1, 1, 1, 1), (5, 3, 3, 9), (100, 1, 17, 100), (300, 2, 40, 7), (3000, 33, 130, 10),
+    (20_000, 50, 64, 1024), (4097, 70, 96, 500), (2000, 129, 256, 64)])
+def test_low_precision_shapes_on_representable_data(gpu, knns_mod, oracle_mod, prec, m, n, d, k):
+    """BF16 and FP8 on integer data |x| <= 15 (exact in both formats, so the shortlist keys are the
+    exact keys): every shape returns the exact answer. Covers one-query waves, k > m (padding:
+    index -1, distance +inf), and shortlists above the merge's register capacity (k = 1024 gives
+    a BF16 shortlist of 1552, FP8 2048: the merge's global-memory select)."""
+    _print_case("lowprec_shapes", prec=prec, m=m, n=n, d=d, k=k)
+    g = np.random.default_rng(m * 7 + d)
+    R = g.integers(-15, 16, (m, d)).astype(np.float32)
+    Q = g.integers(-15, 16, (n, d)).astype(np.float32)
+    kk = min(k, m)
+    D, I = oracle_mod.exact_topk(Q, R, kk)
+    with knns_mod.Corpus(R, bf16=prec == "bf16", fp8=prec == "fp8") as c:
+        dd, ii = c.search(Q, k, precision=prec)
+    assert np.array_equal(ii[:, :kk].astype(np.int64), I.astype(np.int64))
+    assert np.array_equal(dd[:, :kk], D.astype(np.float32))
+    if k > m:
+        assert np.all(ii[:, m:] == -1) and np.all(np.isinf(dd[:, m:]))
