@@ -3,6 +3,14 @@ from pathlib import Path
 
 import pytest
 
+# torch ships its own HIP runtime beside the system one the library links; the two coexist in
+# one process when torch is imported before the library is first loaded (the order of bench.py),
+# so tests that hand torch device tensors to the library see the same device either way
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = Path(__file__).resolve().parent.parent
 for p in (ROOT / "oracle", ROOT / "knn-toolbox_amd", ROOT):
     if str(p) not in sys.path:

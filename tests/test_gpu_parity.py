@@ -185,3 +185,46 @@ def test_low_precision_shapes_on_representable_data(gpu, knns_mod, oracle_mod, p
     assert np.array_equal(dd[:, :kk], D.astype(np.float32))
     if k > m:
         assert np.all(ii[:, m:] == -1) and np.all(np.isinf(dd[:, m:]))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
+def test_paths_on_representable_data(gpu, knns_mod, oracle_mod, prec):
+    """The native API's other paths, per precision, on integer data (exact keys in every format):
+    all-kNN with self exclusion (including duplicated rows), per-query k, device pointers on a
+    caller's stream, double output distances for a float corpus."""
+    import torch
+
+    _print_case("paths", prec=prec)
+    g = np.random.default_rng(17)
+    m, d = 6000, 40
+    R = g.integers(-15, 16, (m, d)).astype(np.float32)
+    R[100:110] = R[7]                      # duplicates: ties go to the lower index, self by index
+    with knns_mod.Corpus(R, bf16=prec == "bf16", fp8=prec == "fp8") as c:
+        # all-kNN over the first 500 rows, self excluded by index
+        D, I = oracle_mod.exact_topk(R[:500], R, 12, self_ids=np.arange(500))
+        dd, ii = c.search(R[:500], 12, precision=prec, exclude_self=True, self_offset=0)
+        assert np.array_equal(ii.astype(np.int64), I.astype(np.int64))
+        assert np.array_equal(dd, D.astype(np.float32))
+        # per-query k
+        Q = g.integers(-15, 16, (90, d)).astype(np.float32)
+        kq = g.choice([1, 5, 40, 130], size=90).astype(np.int32)
+        res = c.search_multi(Q, kq, precision=prec)
+        D, I = oracle_mod.exact_topk(Q, R, 130)
+        for i in range(90):
+            assert np.array_equal(res[i][1].astype(np.int64), I[i, :kq[i]].astype(np.int64)), i
+            assert np.array_equal(res[i][0], D[i, :kq[i]].astype(np.float32)), i
+        # device pointers on a caller's stream
+        dev = torch.device("cuda:0")
+        s = torch.cuda.Stream(device=dev)
+        Qd = torch.from_numpy(Q).to(dev)
+        Dd = torch.empty((90, 33), dtype=torch.float32, device=dev)
+        Id = torch.empty((90, 33), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        c.search_device(Qd, 90, 33, Dd, Id, precision=prec, stream=s.cuda_stream)
+        assert np.array_equal(Id.cpu().numpy().astype(np.int64), I[:, :33].astype(np.int64))
+        assert np.array_equal(Dd.cpu().numpy(), D[:, :33].astype(np.float32))
+        # double distances for a float corpus: the FP64 oracle's values exactly
+        d64, i64 = c.search(Q, 20, precision=prec, dist_f64=True)
+        assert d64.dtype == np.float64
+        assert np.array_equal(i64.astype(np.int64), I[:, :20].astype(np.int64))
+        assert np.array_equal(d64, D[:, :20])
