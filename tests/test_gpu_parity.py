@@ -167,12 +167,14 @@ def test_adversarial_order_forces_in_scan_compaction(gpu, knns_mod, oracle_mod, 
 @pytest.mark.parametrize("prec", ["bf16", "fp8"])
 @pytest.mark.parametrize("m,n,d,k", [
     (1, 1, 1, 1), (5, 3, 3, 9), (100, 1, 17, 100), (300, 2, 40, 7), (3000, 33, 130, 10),
-    (20_000, 50, 64, 1024), (4097, 70, 96, 500), (2000, 129, 256, 64)])
+    (20_000, 50, 64, 1024), (4097, 70, 96, 500), (2000, 129, 256, 64), (2000, 40, 1024, 16)])
 def test_low_precision_shapes_on_representable_data(gpu, knns_mod, oracle_mod, prec, m, n, d, k):
     """BF16 and FP8 on integer data |x| <= 15 (exact in both formats, so the shortlist keys are the
     exact keys): every shape returns the exact answer. Covers one-query waves, k > m (padding:
     index -1, distance +inf), and shortlists above the merge's register capacity (k = 1024 gives
     a BF16 shortlist of 1552, FP8 2048: the merge's global-memory select)."""
+    if prec == "fp8" and d > 256:
+        pytest.skip("FP8 rows are built for d <= 256 (wider corpora are rejected: tests/test_fp8.py)")
     _print_case("lowprec_shapes", prec=prec, m=m, n=n, d=d, k=k)
     g = np.random.default_rng(m * 7 + d)
     R = g.integers(-15, 16, (m, d)).astype(np.float32)
