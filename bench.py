@@ -25,7 +25,6 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "knn-toolbox_amd"))
-sys.path.insert(0, str(ROOT / "oracle"))
 
 BF16_PEAK_TFLOPS = 2516.6
 FP32_PEAK_TFLOPS = 157.3
@@ -45,6 +44,44 @@ def measured_traffic(m: int, n: int, k: int):
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ---- CPU leg: the only part of this script that touches the CPU oracle (oracle/), as the checker
+# of the GPU answers on a query sample and as the timed cpu_baseline ----
+def _oracle():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    return oracle
+
+
+def cpu_check(Qs, R, k, I, D, self_ids=None) -> dict:
+    """Recall@k, largest relative distance error and exact index parity of the GPU answer (I, D)
+    on the sample queries Qs, against the FP64 oracle."""
+    oracle = _oracle()
+    Do, Io = oracle.exact_topk(Qs, R, k, self_ids=self_ids)
+    return {"recall_at_k": oracle.recall_at_k(I, Io), "max_rel_dist_err": oracle.max_rel_err(D, Do),
+            "indices_identical": bool(np.array_equal(I, Io)),
+            "bit_exact": bool(np.array_equal(I, Io) and np.array_equal(D, Do.astype(D.dtype)))}
+
+
+def cpu_baseline(Q, R, k) -> dict:
+    """The knn.m transcription (oracle/knn_oracle.cpp) timed on a bounded query sample on the host
+    cores (at most 16 threads)."""
+    oracle = _oracle()
+    cores = os.cpu_count() or 1
+    try:
+        cores = min(cores, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    cores = min(cores, 16)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    sample = min(Q.shape[0], cores * 96)  # each query computes m keys and stable-sorts them (~0.1 s/core)
+    c0 = time.perf_counter()
+    oracle.knnm(Q[:sample], R, k)
+    cel = time.perf_counter() - c0
+    return {"value": sample / cel, "unit": "queries/s", "cores": cores, "kind": "port",
+            "sample": f"{sample} queries x {R.shape[0]} corpus rows, demo/knn.m transcription (FP32 GEMM, "
+                      f"full stable sort per query), {cel:.1f} s"}
 
 
 def run_serving(knns, corpus, Q, seconds: float, fractions=(0.5,), max_batch: int = 4096,
@@ -136,11 +173,11 @@ def run_serving(knns, corpus, Q, seconds: float, fractions=(0.5,), max_batch: in
     return out
 
 
-def run_c5(args, knns, oracle, torch) -> dict:
+def run_c5(args, knns, workloads, torch) -> dict:
     """C5: serving loop on a resident 20M x 128 BF16 corpus (FP32 master for the rerank),
     16384-centre mixture: saturation, then Poisson arrivals at 0.25/0.5/0.75/0.9 of it."""
     t0 = time.time()
-    R, Q = oracle.workload_c5()
+    R, Q = workloads.workload_c5()
     log(f"C5 generated {R.shape} in {time.time() - t0:.1f}s")
     t0 = time.time()
     corpus = knns.Corpus(R, bf16=True)
@@ -175,12 +212,12 @@ def timed(run, steps: int, warmup: int = 1):
     return (time.perf_counter() - t0) / steps
 
 
-def run_c3(args, knns, oracle, torch) -> dict:
+def run_c3(args, knns, workloads, torch) -> dict:
     """C3: corpus 10M x 96 (4096-centre mixture, sigma 0.15), 10k queries, k=10, shortlist 40 (and 72,
     the FP8 default): q/s and recall@10 / max relative distance error vs the FP64 oracle per precision.
     scan_tflops counts the 2*n*m*d flops of the real dimensions (the FP8 rows are padded to 128)."""
     t0 = time.time()
-    R, Q = oracle.workload_c3()
+    R, Q = workloads.workload_c3()
     m, d = R.shape
     n, k = Q.shape[0], 10
     log(f"C3 generated {R.shape} in {time.time() - t0:.1f}s")
@@ -190,7 +227,6 @@ def run_c3(args, knns, oracle, torch) -> dict:
     Dd = torch.empty((n, k), dtype=torch.float32, device=dev)
     Id = torch.empty((n, k), dtype=torch.int32, device=dev)
     rs = np.random.default_rng(5).choice(n, size=64, replace=False)
-    Do, Io = oracle.exact_topk(Q[rs], R, k)
     out = {}
     peak = {"bf16": 2516.6, "fp8": 5033.2, "fp32": 157.3}
     for name, prec, sl in (("bf16", "bf16", 40), ("fp8", "fp8", 40), ("fp8_shortlist72", "fp8", 72),
@@ -203,11 +239,10 @@ def run_c3(args, knns, oracle, torch) -> dict:
         I = Id.cpu().numpy()
         D = Dd.cpu().numpy()
         tf = 2.0 * n * m * d / st["scan_s"] / 1e12
+        chk = cpu_check(Q[rs], R, k, I[rs], D[rs])
         out[name] = {"qps": n / sec, "ms_per_step": sec * 1e3, "shortlist": sl or k + 16,
                      "scan_ms": st["scan_s"] * 1e3, "scan_tflops": tf, "scan_frac_of_peak": tf / peak[prec],
-                     "recall_at_k": oracle.recall_at_k(I[rs], Io),
-                     "max_rel_dist_err": oracle.max_rel_err(D[rs], Do),
-                     "indices_identical": bool(np.array_equal(I[rs], Io))}
+                     **{c: chk[c] for c in ("recall_at_k", "max_rel_dist_err", "indices_identical")}}
         log(f"C3 {name}: {out[name]}")
     corpus.close()
     return {"metric": "queries/s at exact top-k (corpus 10M x 96, k=10)", "value": out["bf16"]["qps"],
@@ -218,13 +253,13 @@ def run_c3(args, knns, oracle, torch) -> dict:
             "precisions": out}
 
 
-def run_c4(args, knns, oracle, torch) -> dict:
+def run_c4(args, knns, workloads, torch) -> dict:
     """C4: 200k x 32 FP64 U[-1,1), all-kNN excluding self, k=8: the 64 log-uniform batches of
     the workload (sizes rescaled to sum to 200k) form one search over all rows."""
-    X = oracle.workload_c4()
+    X = workloads.workload_c4()
     m, d = X.shape
     k = 8
-    sizes = oracle.log_uniform_sizes(64, 64, 16384, seed=7, total=m)
+    sizes = workloads.log_uniform_sizes(64, 64, 16384, seed=7, total=m)
     assert int(sizes.sum()) == m
     with knns.Corpus(X) as corpus:
         dev = torch.device("cuda:0")
@@ -242,8 +277,7 @@ def run_c4(args, knns, oracle, torch) -> dict:
         I = Id.cpu().numpy()
         D = Dd.cpu().numpy()
     rs = np.random.default_rng(6).choice(m, size=512, replace=False)
-    Do, Io = oracle.exact_topk(X[rs], X, k, self_ids=rs.astype(np.int64))
-    ok = bool(np.array_equal(I[rs], Io) and np.array_equal(D[rs], Do))
+    ok = cpu_check(X[rs], X, k, I[rs], D[rs], self_ids=rs.astype(np.int64))["bit_exact"]
     log(f"C4: {m / sec:.0f} q/s, sample bit-exact vs FP64 oracle: {ok}")
     return {"metric": "all-kNN queries/s (corpus 200k x 32 FP64, self excluded, k=8)", "value": m / sec,
             "unit": "queries/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
@@ -286,18 +320,18 @@ def main() -> int:
     import torch
 
     import knns
-    import oracle
+    import workloads
 
     if knns.device_count() <= 0:
         log("no GPU visible")
         return 3
     if args.workload in ("c3", "c4", "c5"):
-        res = {"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args, knns, oracle, torch)
+        res = {"c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args, knns, workloads, torch)
         print(json.dumps(res))
         return 0
     m, n, d, k = args.m, args.n, 128, args.k
     t0 = time.time()
-    R, Q = oracle.workload_c2(m=m, n=n)
+    R, Q = workloads.workload_c2(m=m, n=n)
     log(f"generated corpus {R.shape} queries {Q.shape} in {time.time() - t0:.1f}s")
 
     corpus = knns.Corpus(R, bf16=True)
@@ -414,30 +448,15 @@ def main() -> int:
     if not args.skip_recall:
         rs = np.random.default_rng(123).choice(n, size=min(n, 128), replace=False)
         r0 = time.time()
-        Do, Io = oracle.exact_topk(Q[rs], R, k)
-        quality = {"recall_at_k": oracle.recall_at_k(I_bf16[rs], Io),
-                   "max_rel_dist_err": oracle.max_rel_err(D_bf16[rs], Do),
-                   "indices_identical": bool(np.array_equal(I_bf16[rs], Io)),
+        chk = cpu_check(Q[rs], R, k, I_bf16[rs], D_bf16[rs])
+        quality = {**{c: chk[c] for c in ("recall_at_k", "max_rel_dist_err", "indices_identical")},
                    "sample_queries": int(len(rs))}
         log(f"quality vs FP64 oracle ({time.time() - r0:.1f}s): {quality}")
 
     # ---- CPU baseline: knn.m transcription on a bounded sample ----
     cpu = None
     if not args.skip_cpu:
-        cores = os.cpu_count() or 1
-        try:
-            cores = min(cores, len(os.sched_getaffinity(0)))
-        except AttributeError:
-            pass
-        cores = min(cores, 16)
-        os.environ["OMP_NUM_THREADS"] = str(cores)
-        sample = min(n, cores * 96)  # each query computes m keys and stable-sorts them (~0.1 s/core)
-        c0 = time.perf_counter()
-        oracle.knnm(Q[:sample], R, k)
-        cel = time.perf_counter() - c0
-        cpu = {"value": sample / cel, "unit": "queries/s", "cores": cores, "kind": "port",
-               "sample": f"{sample} queries x {m} corpus rows, demo/knn.m transcription (FP32 GEMM, "
-                         f"full stable sort per query), {cel:.1f} s"}
+        cpu = cpu_baseline(Q, R, k)
         log(f"cpu baseline: {cpu}")
 
     lat_ms = np.array(lat) * 1e3

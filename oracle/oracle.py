@@ -1,7 +1,9 @@
 """CPU oracle for the brute-force kNN path -- TEST INFRASTRUCTURE ONLY.
 
-Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import this
-module, and only to check results (or as the timed CPU baseline). The GPU product path never does.
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU leg (``cpu_check``: the recall of
+the GPU answer on a query sample; ``cpu_baseline``: the timed CPU baseline) may import this module,
+and only to check results (or as the timed CPU baseline). The GPU product path never does; the
+benchmark's inputs come from ``knn-toolbox_amd/workloads.py``.
 
 Contents
 --------
@@ -10,7 +12,8 @@ Contents
       brute force and the key formulation the GPU code ranks on, include/utils.h:957-985).
     - ``exact_topk``: FP64 brute force with the canonical distance (sequential FMA of squared
       differences) and (dist, index) ordering.
-* Data generators (the synthetic inputs of every workload, written down once, here).
+* Data generators, re-exported from ``knn-toolbox_amd/workloads.py`` (the synthetic inputs of every
+  workload, written down once there).
 * Python restatements of the reference's selection kernels, run on explicit key matrices:
     - ``ref_heap_select``   src/cuknnHeap.cu:65-174 (heap_initialize, heap_insertion),
                             :226-250 (selection), :100-144 (final_unordered), :465 (qk)
@@ -136,79 +139,16 @@ def canonical_dist_np(q: np.ndarray, r: np.ndarray) -> float:
 
 
 # ----------------------------------------------------------------------------------------------
-# Synthetic workloads (the generator definitions every test and the bench use)
+# Synthetic workloads: the generators live beside the library (knn-toolbox_amd/workloads.py, pure
+# numpy) so that the benchmark draws its inputs without this module; the tests reach them here.
 # ----------------------------------------------------------------------------------------------
-def uniform(n: int, d: int, seed: int, lo: float = 0.0, hi: float = 1.0, dtype=np.float32) -> np.ndarray:
-    """i.i.d. U[lo, hi) from numpy PCG64(seed), generated in float64 then rounded to dtype."""
-    g = np.random.Generator(np.random.PCG64(seed))
-    x = g.random((n, d), dtype=np.float64)
-    return (lo + (hi - lo) * x).astype(dtype)
+import sys as _sys
 
-
-def mixture_centers(ncenters: int, d: int, seed: int) -> np.ndarray:
-    g = np.random.Generator(np.random.PCG64(seed))
-    return g.standard_normal((ncenters, d))
-
-
-def mixture(n: int, d: int, ncenters: int, sigma: float, center_seed: int, point_seed: int,
-            dtype=np.float32, chunk: int = 1 << 20) -> np.ndarray:
-    """Equal-weight isotropic Gaussian mixture: centres ~ N(0, I) drawn from PCG64(center_seed);
-    each point picks a centre uniformly and adds sigma * N(0, I), both from PCG64(point_seed).
-    Queries of a workload use the corpus' centres with their own point_seed."""
-    C = mixture_centers(ncenters, d, center_seed)
-    g = np.random.Generator(np.random.PCG64(point_seed))
-    out = np.empty((n, d), dtype=dtype)
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        a = g.integers(0, ncenters, size=e - s)
-        out[s:e] = (C[a] + sigma * g.standard_normal((e - s, d))).astype(dtype)
-    return out
-
-
-def log_uniform_sizes(count: int, lo: int, hi: int, seed: int, total: int | None = None) -> np.ndarray:
-    """Batch sizes log-uniform in [lo, hi]; if total is given, rescaled to sum exactly to total by
-    largest remainder."""
-    g = np.random.Generator(np.random.PCG64(seed))
-    s = np.exp(g.uniform(np.log(lo), np.log(hi), size=count))
-    if total is None:
-        return np.maximum(1, np.round(s)).astype(np.int64)
-    w = s / s.sum() * total
-    base = np.floor(w).astype(np.int64)
-    rem = total - base.sum()
-    order = np.argsort(-(w - base), kind="stable")
-    base[order[:rem]] += 1
-    return base
-
-
-# Workload definitions (corpus/query generators). Each returns (corpus, queries).
-def workload_c1():
-    """m=16384 x d=128 U[0,1) seed 0; n=1024 U[0,1) seed 1; k=16."""
-    return uniform(16384, 128, 0), uniform(1024, 128, 1)
-
-
-def workload_c2(m: int = 1_000_000, n: int = 10_000):
-    """1024-centre mixture, sigma 0.1, d=128: corpus seed 2, queries seed 3 (same centres)."""
-    R = mixture(m, 128, 1024, 0.1, center_seed=2, point_seed=2 + 1000)
-    Q = mixture(n, 128, 1024, 0.1, center_seed=2, point_seed=3 + 1000)
-    return R, Q
-
-
-def workload_c5(m: int = 20_000_000, n: int = 20_000):
-    """Serving corpus: 16384-centre mixture, sigma 0.1, d=128: corpus seed 8; the query pool that
-    requests draw from uses the same centres (seed 10; seed 9 drives the arrival process)."""
-    R = mixture(m, 128, 16384, 0.1, center_seed=8, point_seed=8 + 1000)
-    Q = mixture(n, 128, 16384, 0.1, center_seed=8, point_seed=10 + 1000)
-    return R, Q
-
-
-def workload_c3(m: int = 10_000_000, n: int = 10_000):
-    R = mixture(m, 96, 4096, 0.15, center_seed=4, point_seed=4 + 1000)
-    Q = mixture(n, 96, 4096, 0.15, center_seed=4, point_seed=5 + 1000)
-    return R, Q
-
-
-def workload_c4(m: int = 200_000):
-    return uniform(m, 32, 6, lo=-1.0, hi=1.0, dtype=np.float64)
+_PKG = str(HERE.parent / "knn-toolbox_amd")
+if _PKG not in _sys.path:
+    _sys.path.insert(0, _PKG)
+from workloads import (log_uniform_sizes, mixture, mixture_centers, uniform, workload_c1,  # noqa: E402,F401
+                       workload_c2, workload_c3, workload_c4, workload_c5)
 
 
 # ----------------------------------------------------------------------------------------------
