@@ -1081,6 +1081,110 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
 // one LDS read of a per-entry source map instead of a binary search over the source offsets.
 constexpr int MR_U = 16;
 
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+// K-th smallest (1 <= K <= N) of the N distinct entries (key << 32 | row) held by one wave in
+// registers: v[u] of lane l is entry l + 64 u, for l + 64 u < N. Radix select over the 32-bit keys
+// (8 bits per level, one LDS histogram, the chosen bin broadcast by readlane), so every level costs
+// one pass of 32-bit arithmetic and two wave syncs; entries that share the K-th key are resolved
+// by the 64-bit select over that key's entries only. hist: >= 256 + 16 uint32 of LDS.
+template <int U>
+__device__ uint64_t wave_kth_regs(const uint64_t (&v)[U], int N, int K, uint32_t* hist) {
+  const int lane = threadIdx.x & 63;
+  uint32_t kmn = ~0u, kmx = 0u;
+  uint64_t emn = ~0ull, emx = 0ull;
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (lane + 64 * u < N) {
+      const uint32_t key = (uint32_t)(v[u] >> 32);
+      kmn = min(kmn, key);
+      kmx = max(kmx, key);
+    }
+  kmn = wave_min_u32(kmn);
+  kmx = wave_max_u32(kmx);
+  if (K >= N || K <= 1) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (lane + 64 * u < N) {
+        emn = v[u] < emn ? v[u] : emn;
+        emx = v[u] > emx ? v[u] : emx;
+      }
+    return K >= N ? wave_max_u64(emx) : wave_min_u64(emn);
+  }
+  *(u32x4*)(hist + 4 * lane) = (u32x4){0u, 0u, 0u, 0u};
+  wave_sync();
+  uint32_t base = kmn;
+  int below = 0;
+  const uint32_t span = kmx - kmn;
+  int sh = span ? 32 - __clz((int)span) : 0;
+  sh = sh > 8 ? sh - 8 : 0;
+  for (;;) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (lane + 64 * u < N) {
+        const uint32_t key = (uint32_t)(v[u] >> 32);
+        const uint32_t r = (key - base) >> sh;
+        if (key >= base && r < 256u) atomicAdd(&hist[r], 1u);
+      }
+    wave_sync();
+    const u32x4 c = *(const u32x4*)(hist + 4 * lane);
+    *(u32x4*)(hist + 4 * lane) = (u32x4){0u, 0u, 0u, 0u};
+    wave_sync();
+    const uint32_t tot = c[0] + c[1] + c[2] + c[3];
+    const uint32_t incl = wave_incl_scan_u32(tot, lane);
+    const uint32_t need = (uint32_t)(K - below);
+    const int fl = __ffsll((long long)__builtin_amdgcn_ballot_w64(incl >= need)) - 1;
+    // lane fl holds the bin with the K-th key: its index, the count below it and its count
+    uint32_t acc = incl - tot, b = 0, cb = c[0];
+    if (acc + c[0] < need) {
+      acc += c[0]; b = 1; cb = c[1];
+      if (acc + c[1] < need) {
+        acc += c[1]; b = 2; cb = c[2];
+        if (acc + c[2] < need) { acc += c[2]; b = 3; cb = c[3]; }
+      }
+    }
+    const uint32_t bstar = (uint32_t)__builtin_amdgcn_readlane((int)(4 * lane + b), fl);
+    below += __builtin_amdgcn_readlane((int)acc, fl);
+    const int cbin = __builtin_amdgcn_readlane((int)cb, fl);
+    base += bstar << sh;
+    if (below + cbin == K || below + 1 == K || sh == 0) {
+      // below + cbin == K: every entry of the bin is kept and the K-th is the bin's largest;
+      // below + 1 == K: the K-th is the bin's smallest.
+      // sh == 0: the bin is one key; the K-th is the (K - below)-th of its entries by row.
+      const int want = K - below;
+      uint64_t m2 = 0ull, m1 = ~0ull;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (lane + 64 * u < N) {
+          const uint32_t key = (uint32_t)(v[u] >> 32);
+          if (key >= base && ((key - base) >> sh) == 0u) {
+            m2 = v[u] > m2 ? v[u] : m2;
+            m1 = v[u] < m1 ? v[u] : m1;
+          }
+        }
+      if (want == cbin) return wave_max_u64(m2);
+      if (want == 1) return wave_min_u64(m1);
+      const uint32_t kk = base;
+      auto each = [&](auto f) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (lane + 64 * u < N && (uint32_t)(v[u] >> 32) == kk) f(v[u]);
+      };
+      return group_kth_each<true>(each, cbin, want, hist);
+    }
+    sh = sh > 8 ? sh - 8 : 0;
+  }
+}
+
 __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
   __shared__ int off_all[4][MERGE_MAX_SRC + 1];
   __shared__ uint16_t map_all[4][64 * MR_U];
@@ -1168,12 +1272,7 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
         v[u] = key_of(s, i - off[s]);
       }
     }
-    auto each = [&](auto f) {
-#pragma unroll
-      for (int u = 0; u < MR_U; ++u)
-        if (lane + 64 * u < N) f(v[u]);
-    };
-    if (N > K) E = group_kth_each<true>(each, N, K, kth_all[wv]);
+    if (N > K) E = wave_kth_regs<MR_U>(v, N, K, kth_all[wv]);
 #pragma unroll
     for (int u = 0; u < MR_U; ++u) {
       if (64 * u >= N) break;   // wave-uniform
