@@ -574,7 +574,8 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   // per-query k: staged through pinned host memory (no synchronisation inside the search)
   if (!uniform) {
     void* pin;
-    if ((sst = E.pinned(n * 16, &pin)) != KNNS_OK) return sst;
+    // k and shortlist (4 bytes each per query), padded to 8 bytes, then the offsets (8 bytes each)
+    if ((sst = E.pinned(n * 16 + 8, &pin)) != KNNS_OK) return sst;
     int* pk = (int*)pin;
     int* pc = pk + n;
     long* po = (long*)(pc + n + (n & 1));
