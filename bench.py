@@ -302,6 +302,8 @@ def main() -> int:
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-host-io", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--skip-variants", action="store_true",
+                    help="skip the entry-point / k variants (threshold and bitonic, k in {10, 100})")
     ap.add_argument("--serving-max-batch", type=int, default=4096,
                     help="batch cap (largest bucket) of the headline serving phase")
     ap.add_argument("--serving-seconds", type=float, default=3.0,
@@ -406,7 +408,7 @@ def main() -> int:
     # ---- the workload's other variants: both entry points (threshold = heap, bitonic) and
     # k in {10, 100}, each checked against the k=100 threshold answer (its first k columns) ----
     variants = None
-    if not args.skip_fp32 and k == 100:
+    if not args.skip_fp32 and not args.skip_variants and k == 100:
         variants = {}
         for prec in ("fp32", "bf16"):
             for sel in ("threshold", "bitonic"):
