@@ -230,3 +230,50 @@ def test_paths_on_representable_data(gpu, knns_mod, oracle_mod, prec):
         assert d64.dtype == np.float64
         assert np.array_equal(i64.astype(np.int64), I[:, :20].astype(np.int64))
         assert np.array_equal(d64, D[:, :20])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_metamorphic_invariants(gpu, knns_mod, oracle_mod, prec):
+    """Properties the exact answer has whatever the oracle says, on continuous data (no exact
+    distance ties): (1) permuting the corpus rows permutes the returned indices and leaves the
+    distances unchanged; (2) scaling queries and corpus by 2^j scales every distance by 4^j
+    exactly (BF16 and FP32 keys, the FP64 rerank and the final rounding all commute with a power
+    of two) and leaves the indices unchanged; (3) k=1 returns the argmin of the FP64 distances;
+    (4) k=m returns the whole corpus sorted by (distance, index)."""
+    _print_case("metamorphic", prec=prec)
+    g = np.random.default_rng(23)
+    m, n, d, k = 3000, 70, 40, 16
+
+    def search(Q, R, kk):
+        if prec == "fp32":
+            dist, idx, _ = knns_mod.gpuknns_heap(Q, R, kk)
+            return dist, idx.astype(np.int64)
+        with knns_mod.Corpus(R, bf16=True) as c:
+            dist, idx = c.search(Q, kk, precision="bf16")
+        return dist, idx.astype(np.int64)
+
+    R = g.standard_normal((m, d)).astype(np.float32)
+    Q = g.standard_normal((n, d)).astype(np.float32)
+    D0, I0 = search(Q, R, k)
+    Do, Io = oracle_mod.exact_topk(Q, R, k)
+    assert np.array_equal(I0, Io.astype(np.int64)) and np.array_equal(D0, Do.astype(np.float32))
+    # (1) permutation of the corpus
+    p = g.permutation(m)
+    D1, I1 = search(Q, R[p], k)
+    assert np.array_equal(p[I1], I0) and np.array_equal(D1, D0)
+    # (2) scaling by powers of two (no overflow or underflow at these magnitudes)
+    for j in (3, -5):
+        s = np.float32(2.0 ** j)
+        D2, I2 = search(Q * s, R * s, k)
+        assert np.array_equal(I2, I0), j
+        assert np.array_equal(D2, D0 * np.float32(4.0 ** j)), j
+    # (3) k = 1 is the argmin of the FP64 distances
+    full = ((Q.astype(np.float64)[:, None, :] - R.astype(np.float64)[None, :, :]) ** 2).sum(-1)
+    D3, I3 = search(Q, R, 1)
+    assert np.array_equal(I3[:, 0], np.argmin(full, axis=1))
+    # (4) k = m on a small corpus: every row, sorted by (distance, index)
+    Rs = R[:200]
+    D4, I4 = search(Q[:10], Rs, 200)
+    Df, If = oracle_mod.exact_topk(Q[:10], Rs, 200)
+    assert np.array_equal(I4, If.astype(np.int64)) and np.array_equal(D4, Df.astype(np.float32))
+    assert all(sorted(r.tolist()) == list(range(200)) for r in I4)
