@@ -44,7 +44,8 @@ _lib = None
 def build(force: bool = False) -> Path:
     """Compile knn_oracle.cpp into oracle/_build/liboracle.so (g++ -O3 -fopenmp)."""
     if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "knn_oracle.cpp").stat().st_mtime:
-        subprocess.run(["make", "-s", "-C", str(HERE), "lib"], check=True)
+        # force: rebuild even when make considers the library up to date
+        subprocess.run(["make", "-s", *(["-B"] if force else []), "-C", str(HERE), "lib"], check=True)
     return LIB
 
 
