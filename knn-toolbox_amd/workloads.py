@@ -31,8 +31,10 @@ def mixture_centers(ncenters: int, d: int, seed: int) -> np.ndarray:
 def mixture(n: int, d: int, ncenters: int, sigma: float, center_seed: int, point_seed: int,
             dtype=np.float32, chunk: int = 1 << 20) -> np.ndarray:
     """Equal-weight isotropic Gaussian mixture: centres ~ N(0, I) drawn from PCG64(center_seed);
-    each point picks a centre uniformly and adds sigma * N(0, I), both from PCG64(point_seed).
-    Queries of a workload use the corpus' centres with their own point_seed."""
+    each point picks a centre uniformly and adds sigma * N(0, I), both from PCG64(point_seed),
+    in chunks of `chunk` points (each chunk draws its centre picks, then its noise: the chunk size
+    is part of the definition). Queries of a workload use the corpus' centres with their own
+    point_seed."""
     C = mixture_centers(ncenters, d, center_seed)
     g = np.random.Generator(np.random.PCG64(point_seed))
     out = np.empty((n, d), dtype=dtype)
