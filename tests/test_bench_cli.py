@@ -26,7 +26,7 @@ def test_workloads_are_seeded_and_shared_with_the_oracle(oracle_mod):
     a = workloads.mixture(1000, 16, 32, 0.1, center_seed=2, point_seed=1002)
     b = workloads.mixture(1000, 16, 32, 0.1, center_seed=2, point_seed=1002)
     assert np.array_equal(a, b) and a.dtype == np.float32
-    # the C2 generator at a reduced size is a prefix-independent draw of the same mixture
+    # the C2 generator at a reduced size is the C2 mixture definition (centre seed 2, point seed 1002)
     R, Q = workloads.workload_c2(m=2000, n=50)
     assert R.shape == (2000, 128) and Q.shape == (50, 128)
     assert np.array_equal(R, workloads.mixture(2000, 128, 1024, 0.1, center_seed=2, point_seed=1002))
