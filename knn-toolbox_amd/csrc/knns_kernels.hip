@@ -817,10 +817,21 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
     }
   };
   // all loads of a stage, issued right after the barrier (spreading them over the k-steps was
-  // measured: no gain for BF16, 4% slower for FP8)
+  // measured: no gain for BF16, 4% slower for FP8). The main loads step through the rows with a
+  // pointer add each (one 64-bit multiply per stage instead of one per load).
+  const size_t kstride = (size_t)(4 * G::RPL) * ldb;   // bytes between a wave's main loads
   auto issue = [&](int t0, int buf) {
+    const uint8_t* p = X + (size_t)(t0 + wv * G::RPL) * ldb;
 #pragma unroll
-    for (int k = 0; k <= G::PER_WAVE; ++k) issue_one(t0, buf, k);
+    for (int k = 0; k < G::PER_WAVE; ++k) {
+      const uint8_t* pk = p;
+      lds_u8* l = (lds_u8*)(smem + buf * G::MAIN_BYTES + wv * 1024 + k * 4096);
+      asm volatile("" : "+s"(pk), "+s"(l));
+      __builtin_amdgcn_global_load_lds((const void*)(pk + dma_off), (void __attribute__((address_space(3)))*)l,
+                                       16, 0, 0);
+      p += kstride;
+    }
+    issue_one(t0, buf, G::PER_WAVE);
   };
 
   // ---- per-wave query operand (B), resident for the whole slice ----
