@@ -817,19 +817,24 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
     }
   };
   // all loads of a stage, issued right after the barrier (spreading them over the k-steps was
-  // measured: no gain for BF16, 4% slower for FP8). The main loads step through the rows with a
-  // pointer add each (one 64-bit multiply per stage instead of one per load).
-  const size_t kstride = (size_t)(4 * G::RPL) * ldb;   // bytes between a wave's main loads
+  // measured: no gain for BF16, 4% slower for FP8). The main loads are buffer loads into LDS
+  // through one descriptor per stage (base = the stage's first row, range = its SR rows, so no
+  // access can leave the stage): a fixed per-lane offset and a scalar offset per load, three
+  // instructions per load instead of a 64-bit address computation each.
+  const int kstride = (int)(4 * G::RPL) * (int)ldb;   // bytes between a wave's main loads
+  const int voff = (int)(dma_off + (uint32_t)(wv * G::RPL) * ldb);
   auto issue = [&](int t0, int buf) {
-    const uint8_t* p = X + (size_t)(t0 + wv * G::RPL) * ldb;
+    // word 3 = 0x00020000: 32-bit data format, the raw-buffer setting of CDNA descriptors
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(X + (size_t)t0 * ldb), (short)0, (int)(G::SR * ldb), 0x00020000);
+    int soff = 0;
 #pragma unroll
     for (int k = 0; k < G::PER_WAVE; ++k) {
-      const uint8_t* pk = p;
       lds_u8* l = (lds_u8*)(smem + buf * G::MAIN_BYTES + wv * 1024 + k * 4096);
-      asm volatile("" : "+s"(pk), "+s"(l));
-      __builtin_amdgcn_global_load_lds((const void*)(pk + dma_off), (void __attribute__((address_space(3)))*)l,
-                                       16, 0, 0);
-      p += kstride;
+      asm volatile("" : "+s"(l));
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (void __attribute__((address_space(3)))*)l, 16, voff, soff,
+                                               0, 0);
+      soff += kstride;
     }
     issue_one(t0, buf, G::PER_WAVE);
   };
