@@ -798,31 +798,16 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
   const uint32_t dma_off = (uint32_t)(lane / SU) * ldb + (uint32_t)((lane % SU) ^ swl) * 16u;
   const uint32_t aug_off = (uint32_t)lane * ldb + (uint32_t)a.dp * ESZ;
   typedef __attribute__((address_space(3))) uint8_t lds_u8;
-  // Load k of a stage (k < PER_WAVE: main units of rows wv*RPL + 4*RPL*k; k == PER_WAVE: the
-  // augmentation units of rows 64*wv, waves 0 and 1 only). Scalar source and LDS addresses (the
-  // empty asm keeps them in SGPRs instead of per-load 64-bit vector addresses).
-  auto issue_one = [&](int t0, int buf, int k) {
-    if (k < G::PER_WAVE) {
-      const uint8_t* p = X + (size_t)(t0 + wv * G::RPL + k * 4 * G::RPL) * ldb;
-      lds_u8* l = (lds_u8*)(smem + buf * G::MAIN_BYTES + wv * 1024 + k * 4096);
-      asm volatile("" : "+s"(p), "+s"(l));
-      __builtin_amdgcn_global_load_lds((const void*)(p + dma_off), (void __attribute__((address_space(3)))*)l,
-                                       16, 0, 0);
-    } else if (wv < G::SR / 64) {
-      const uint8_t* pa = X + (size_t)(t0 + wv * 64) * ldb;
-      lds_u8* la = (lds_u8*)(smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES + wv * 1024);
-      asm volatile("" : "+s"(pa), "+s"(la));
-      __builtin_amdgcn_global_load_lds((const void*)(pa + aug_off), (void __attribute__((address_space(3)))*)la,
-                                       16, 0, 0);
-    }
-  };
-  // all loads of a stage, issued right after the barrier (spreading them over the k-steps was
-  // measured: no gain for BF16, 4% slower for FP8). The main loads are buffer loads into LDS
-  // through one descriptor per stage (base = the stage's first row, range = its SR rows, so no
-  // access can leave the stage): a fixed per-lane offset and a scalar offset per load, three
-  // instructions per load instead of a 64-bit address computation each.
+  // All loads of a stage, issued right after the barrier (spreading them over the k-steps was
+  // measured: no gain for BF16, 4% slower for FP8): buffer loads into LDS through one descriptor
+  // per stage (base = the stage's first row, range = its SR rows, so no access can leave the
+  // stage), a fixed per-lane offset and a scalar offset per load -- three instructions per load.
+  // Load k < PER_WAVE carries the main units of rows wv*RPL + 4*RPL*k; waves 0 and 1 then load
+  // the augmentation units of rows 64*wv .. 64*wv + 63. The empty asm keeps the LDS addresses
+  // in SGPRs.
   const int kstride = (int)(4 * G::RPL) * (int)ldb;   // bytes between a wave's main loads
   const int voff = (int)(dma_off + (uint32_t)(wv * G::RPL) * ldb);
+  const int aoff = (int)(aug_off + (uint32_t)(wv * 64) * ldb);
   auto issue = [&](int t0, int buf) {
     // word 3 = 0x00020000: 32-bit data format, the raw-buffer setting of CDNA descriptors
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
@@ -836,7 +821,11 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
                                                0, 0);
       soff += kstride;
     }
-    issue_one(t0, buf, G::PER_WAVE);
+    if (wv < G::SR / 64) {
+      lds_u8* la = (lds_u8*)(smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES + wv * 1024);
+      asm volatile("" : "+s"(la));
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (void __attribute__((address_space(3)))*)la, 16, aoff, 0, 0, 0);
+    }
   };
 
   // ---- per-wave query operand (B), resident for the whole slice ----
