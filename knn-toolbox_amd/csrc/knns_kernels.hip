@@ -583,7 +583,10 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   auto load_stage = [&](long s, int buf) {
     const long t0 = c0 + (s / nch) * SR;
     const int c = (int)(s % nch);
-    const T* stage = X + t0 * a.ldx;
+    // one buffer descriptor per stage (base = the stage's first row, range = its SR rows): no
+    // access can leave the stage, and each load is a 32-bit lane offset instead of an address
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(X + t0 * a.ldx), (short)0, (int)((uint32_t)SR * ldx32 * (uint32_t)sizeof(T)), 0x00020000);
     // recompute the per-lane offsets at every stage instead of keeping WLOADS of them live across
     // the loop (the empty asm hides that `ln` equals the loop-invariant lane id)
     int ln = lane;
@@ -595,8 +598,8 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
       const uint32_t r = (uint32_t)(U / G::SU), u = (uint32_t)(U % G::SU);
       const uint32_t off = r * ldx32 + (u < G::MAIN_UNITS ? (uint32_t)(c * DC) + u * G::VEC
                                       : (u < G::UNITS ? (uint32_t)a.dp + (u - G::MAIN_UNITS) * G::VEC : 0u));
-      __builtin_amdgcn_global_load_lds((const void*)(stage + off),
-          (void __attribute__((address_space(3)))*)&tile[buf][blk * 256], 16, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (void __attribute__((address_space(3)))*)&tile[buf][blk * 256], 16,
+                                               (int)(off * (uint32_t)sizeof(T)), 0, 0, 0);
     }
   };
 
