@@ -1270,15 +1270,34 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
       }
     }
     wave_sync();
+    // branch-free: every lane computes a valid address for each slot (idle slots read the first
+    // carry entry), so all MR_U loads are in flight together and one wait covers them; the key
+    // conversion follows in a second pass
     uint64_t v[MR_U];
+    int src[MR_U], jj[MR_U];
+    // the map holds 64 MR_U slots and off[0] always exists: both reads are unconditional (slots
+    // past N read stale values that the selects discard), so they too are in flight together
 #pragma unroll
     for (int u = 0; u < MR_U; ++u) {
-      const int i = lane + 64 * u;
-      v[u] = ~0ull;
-      if (i < N) {
-        const int s = map[i];
-        v[u] = key_of(s, i - off[s]);
-      }
+      const int m = (int)map[lane + 64 * u];
+      src[u] = lane + 64 * u < N ? m : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < MR_U; ++u) jj[u] = off[src[u] < 0 ? 0 : src[u]];
+    __builtin_amdgcn_sched_barrier(0);   // every offset read is issued before its first use
+#pragma unroll
+    for (int u = 0; u < MR_U; ++u) jj[u] = src[u] < 0 ? 0 : lane + 64 * u - jj[u];
+#pragma unroll
+    for (int u = 0; u < MR_U; ++u) {
+      const int s = src[u] < 0 ? 0 : src[u];
+      const uint64_t* p = s == 0 ? cin + jj[u] : sub + (long)(s - 1) * a.sub_cap + jj[u];
+      v[u] = *p;
+    }
+#pragma unroll
+    for (int u = 0; u < MR_U; ++u) {
+      const uint64_t e = v[u];
+      v[u] = src[u] < 0 ? ~0ull
+             : (src[u] == 0 ? e : ((uint32_t)e == self ? ~0ull : pool_key(e)));
     }
     if (N > K) E = wave_kth_regs<MR_U>(v, N, K, kth_all[wv]);
 #pragma unroll
