@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   const T* X = (const T*)a.X;
   const T* Q = (const T*)a.Qp;
 
-  // ---- stage loader: global -> LDS by DMA (global_load_lds_dwordx4) ----
+  // ---- stage loader: global -> LDS by DMA (buffer_load_dwordx4 ... lds) ----
   // The buffer is a dense array of 16-byte units, SU per row. Load l of wave wv fills units
   // [(wv + 4l)*64, +64): lane -> (row, unit) -> global source; padding units re-read the row start.
   // Blocks past the end repeat the last block (same source, destination and bytes) so every wave
@@ -735,12 +735,14 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
 //     at position u ^ sw(r), sw(r) = (r / (16/SU)) % SU, so the 16-byte MFMA operand reads of 16
 //     consecutive rows (8 with 32 banks) touch distinct banks;
 //   * the augmentation units (16 bytes per row) go to a separate LDS array;
-//   * a wave-wide DMA load fills 64 consecutive units = 64/SU whole rows, so with load L of wave
-//     wv being L = wv + 4l, a lane's source offset inside the stage is the same for every load
-//     and every stage (one register); loads differ only by a scalar row offset;
+//   * a wave-wide DMA load (a buffer load into LDS through the stage's descriptor) fills 64
+//     consecutive units = 64/SU whole rows, so with load L of wave wv being L = wv + 4l, a lane's
+//     offset inside the stage is the same for every load and every stage (one register); loads
+//     differ only by a scalar offset;
 //   * operand reads for step j are at (A0 ^ 16j) + immediate, one XOR per step;
-//   * the first MFMA of each accumulator is the augmentation step with a zero C operand, so no
-//     accumulator is ever cleared, and operands are read one step ahead of their MFMAs.
+//   * the augmentation step (one MFMA per 32-row half with a zero C operand) is the first term of
+//     every accumulator, so no accumulator is ever cleared; operands are read one step ahead of
+//     their MFMAs.
 // BF16 rows (SU = dp/8): one 32x32x16 MFMA per unit pair; lane half h takes units h*SU/2 + j.
 // FP8 rows (OCP E4M3, SU = dp8/16): one scaled 32x32x64 MFMA per 4 units; lane half h takes units
 // h*SU/2 + 2j and + 2j + 1 (32 bytes). The E8M0 row scales (2^e_r of the corpus row, 2^e_q of the
