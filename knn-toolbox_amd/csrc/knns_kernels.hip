@@ -1045,13 +1045,28 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
   uint64_t E;
   int kept;
   if (N <= MW_ENT) {
-    // gather: independent loads, four per lane in flight
+    // gather, branch-free: every slot computes a valid address (slots past N read the first
+    // carry entry), so the four loads per lane are in flight together; conversion afterwards
     for (int base = 0; base < N; base += 256) {
       uint64_t v[4];
+      int src[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int i = base + u * 64 + lane;
-        v[u] = i < N ? gget(i) : 0;
+        const int ii = i < N ? i : 0;
+        int lo = 0, hi = nsrc - 1;     // last source whose offset is <= ii
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (off[mid] <= ii) lo = mid; else hi = mid - 1;
+        }
+        const int jj = ii - off[lo];
+        src[u] = i < N ? lo : -1;
+        v[u] = *(lo == 0 ? cin + jj : sub + (long)(lo - 1) * a.sub_cap + jj);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t e = v[u];
+        v[u] = src[u] <= 0 ? e : ((uint32_t)e == self ? ~0ull : pool_key(e));
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
