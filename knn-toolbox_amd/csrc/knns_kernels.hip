@@ -970,6 +970,77 @@ __device__ __forceinline__ void wave_bitonic_u64(uint64_t* v, int P) {
   wave_sync();
 }
 
+__device__ __forceinline__ void ce_u64(uint64_t& x, uint64_t& y, bool up) {
+  const bool sw = (x > y) == up;
+  const uint64_t lo = sw ? y : x;
+  y = sw ? x : y;
+  x = lo;
+}
+
+// Ascending bitonic sort of P (a power of two, 64 <= P <= 1024) u64 entries in LDS by one wave: the same network as
+// wave_bitonic_u64, with every stage of stride < 8 run in registers on 8-entry blocks (block b =
+// entries 8b .. 8b+7, read and written as 16-byte units) -- one block pass per level instead of
+// three LDS stages, so a 1024-entry sort makes 28 LDS stages and 8 block passes instead of 55
+// LDS stages.
+__device__ void wave_bitonic_u64_r8(uint64_t* v, int P) {
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+  const int lane = threadIdx.x & 63;
+  const int nblk = P >> 3;
+  auto load8 = [&](int b, uint64_t (&r)[8]) {
+    const u64x2* p = (const u64x2*)(v + 8 * b);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const u64x2 t = p[k];
+      r[2 * k] = t[0];
+      r[2 * k + 1] = t[1];
+    }
+  };
+  auto store8 = [&](int b, const uint64_t (&r)[8]) {
+    u64x2* p = (u64x2*)(v + 8 * b);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p[k] = (u64x2){r[2 * k], r[2 * k + 1]};
+  };
+  // strides 4, 2, 1 of one level whose direction is uniform over the block
+  auto tail3 = [&](uint64_t (&r)[8], bool up) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ce_u64(r[i], r[i + 4], up);
+#pragma unroll
+    for (int i : {0, 1, 4, 5}) ce_u64(r[i], r[i + 2], up);
+#pragma unroll
+    for (int i : {0, 2, 4, 6}) ce_u64(r[i], r[i + 1], up);
+  };
+  wave_sync();
+  // levels 2, 4 and 8 lie inside the blocks
+  for (int b = lane; b < nblk; b += 64) {
+    uint64_t r[8];
+    load8(b, r);
+    ce_u64(r[0], r[1], true); ce_u64(r[2], r[3], false); ce_u64(r[4], r[5], true); ce_u64(r[6], r[7], false);
+    ce_u64(r[0], r[2], true); ce_u64(r[1], r[3], true); ce_u64(r[4], r[6], false); ce_u64(r[5], r[7], false);
+    ce_u64(r[0], r[1], true); ce_u64(r[2], r[3], true); ce_u64(r[4], r[5], false); ce_u64(r[6], r[7], false);
+    tail3(r, (b & 1) == 0);
+    store8(b, r);
+  }
+  for (int size = 16; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride >= 8; stride >>= 1) {
+      wave_sync();
+      for (int t = lane; t < P / 2; t += 64) {
+        const int pos = 2 * t - (t & (stride - 1));
+        const bool up = ((pos & size) == 0);
+        const uint64_t x = v[pos], y = v[pos + stride];
+        if ((x > y) == up) { v[pos] = y; v[pos + stride] = x; }
+      }
+    }
+    wave_sync();
+    for (int b = lane; b < nblk; b += 64) {
+      uint64_t r[8];
+      load8(b, r);
+      tail3(r, ((8 * b) & size) == 0);
+      store8(b, r);
+    }
+  }
+  wave_sync();
+}
+
 __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
   __shared__ __attribute__((aligned(16))) uint64_t ent_all[4][MW_ENT];
   __shared__ int off_all[4][MERGE_MAX_SRC + 1];
@@ -1079,7 +1150,7 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
       int P = 64;
       while (P < N) P <<= 1;
       for (int i = N + lane; i < P; i += 64) ent[i] = ~0ull;
-      wave_bitonic_u64(ent, P);
+      wave_bitonic_u64_r8(ent, P);   // P >= 64
       for (int i = lane; i < K; i += 64) cout[i] = ent[i];
       E = ent[K - 1];
       kept = K;
