@@ -1510,10 +1510,71 @@ __global__ __launch_bounds__(256) void final_block_kernel(FinalArgs a) {
 // candidates at a time: four independent FMA chains, each in the canonical sequential order.
 constexpr int FW_MAX = 512;
 
-__device__ __forceinline__ void wave_bitonic_pair(uint64_t* k, uint32_t* v, int P) {
+__device__ __forceinline__ void ce_pair(uint64_t& xk, uint32_t& xv, uint64_t& yk, uint32_t& yv, bool up) {
+  const bool gt = (xk > yk) || (xk == yk && xv > yv);
+  const bool sw = gt == up;
+  const uint64_t lk = sw ? yk : xk;
+  const uint32_t lv = sw ? yv : xv;
+  yk = sw ? xk : yk;
+  yv = sw ? xv : yv;
+  xk = lk;
+  xv = lv;
+}
+
+// Ascending wave bitonic sort of P (a power of two, 64 <= P <= 512) (key, value) pairs in LDS by
+// (key, value). Stages of stride >= 8 run through LDS; every stage of stride < 8 runs in registers
+// on 8-entry blocks (keys and values read and written as 16-byte units).
+__device__ void wave_bitonic_pair_r8(uint64_t* k, uint32_t* v, int P) {
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
   const int lane = threadIdx.x & 63;
-  for (int size = 2; size <= P; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+  const int nblk = P >> 3;
+  auto load8 = [&](int b, uint64_t (&rk)[8], uint32_t (&rv)[8]) {
+    const u64x2* pk = (const u64x2*)(k + 8 * b);
+    const u32x4* pv = (const u32x4*)(v + 8 * b);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const u64x2 t = pk[i];
+      rk[2 * i] = t[0];
+      rk[2 * i + 1] = t[1];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const u32x4 t = pv[i];
+      rv[4 * i] = t[0]; rv[4 * i + 1] = t[1]; rv[4 * i + 2] = t[2]; rv[4 * i + 3] = t[3];
+    }
+  };
+  auto store8 = [&](int b, const uint64_t (&rk)[8], const uint32_t (&rv)[8]) {
+    u64x2* pk = (u64x2*)(k + 8 * b);
+    u32x4* pv = (u32x4*)(v + 8 * b);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pk[i] = (u64x2){rk[2 * i], rk[2 * i + 1]};
+#pragma unroll
+    for (int i = 0; i < 2; ++i) pv[i] = (u32x4){rv[4 * i], rv[4 * i + 1], rv[4 * i + 2], rv[4 * i + 3]};
+  };
+  auto ce = [&](uint64_t (&rk)[8], uint32_t (&rv)[8], int i, int j, bool up) {
+    ce_pair(rk[i], rv[i], rk[j], rv[j], up);
+  };
+  auto tail3 = [&](uint64_t (&rk)[8], uint32_t (&rv)[8], bool up) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ce(rk, rv, i, i + 4, up);
+#pragma unroll
+    for (int i : {0, 1, 4, 5}) ce(rk, rv, i, i + 2, up);
+#pragma unroll
+    for (int i : {0, 2, 4, 6}) ce(rk, rv, i, i + 1, up);
+  };
+  wave_sync();
+  for (int b = lane; b < nblk; b += 64) {
+    uint64_t rk[8];
+    uint32_t rv[8];
+    load8(b, rk, rv);
+    ce(rk, rv, 0, 1, true); ce(rk, rv, 2, 3, false); ce(rk, rv, 4, 5, true); ce(rk, rv, 6, 7, false);
+    ce(rk, rv, 0, 2, true); ce(rk, rv, 1, 3, true); ce(rk, rv, 4, 6, false); ce(rk, rv, 5, 7, false);
+    ce(rk, rv, 0, 1, true); ce(rk, rv, 2, 3, true); ce(rk, rv, 4, 5, false); ce(rk, rv, 6, 7, false);
+    tail3(rk, rv, (b & 1) == 0);
+    store8(b, rk, rv);
+  }
+  for (int size = 16; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride >= 8; stride >>= 1) {
       wave_sync();
       for (int t = lane; t < P / 2; t += 64) {
         const int pos = 2 * t - (t & (stride - 1));
@@ -1524,13 +1585,21 @@ __device__ __forceinline__ void wave_bitonic_pair(uint64_t* k, uint32_t* v, int 
         if (gt == up) { k[pos] = y; k[pos + stride] = x; v[pos] = yv; v[pos + stride] = xv; }
       }
     }
+    wave_sync();
+    for (int b = lane; b < nblk; b += 64) {
+      uint64_t rk[8];
+      uint32_t rv[8];
+      load8(b, rk, rv);
+      tail3(rk, rv, ((8 * b) & size) == 0);
+      store8(b, rk, rv);
+    }
   }
   wave_sync();
 }
 
 __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
   __shared__ __attribute__((aligned(16))) uint64_t dk_all[4][FW_MAX];
-  __shared__ uint32_t di_all[4][FW_MAX];
+  __shared__ __attribute__((aligned(16))) uint32_t di_all[4][FW_MAX];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = blockIdx.x * 4 + wv;
   if (q >= (a.n_dev ? *a.n_dev : a.n)) return;   // wave-uniform
@@ -1627,7 +1696,7 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
   while (P < Kq) P <<= 1;
   wave_sync();
   for (int i = Kq + lane; i < P; i += 64) { dk[i] = ~0ull; di[i] = ~0u; }
-  wave_bitonic_pair(dk, di, P);
+  wave_bitonic_pair_r8(dk, di, P);   // 64 <= P <= FW_MAX
   const long o = a.out_off[q];
   for (int t = lane; t < k; t += 64) {
     double dist;
