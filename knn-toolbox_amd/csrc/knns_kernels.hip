@@ -1192,7 +1192,7 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // registers: v[u] of lane l is entry l + 64 u, for l + 64 u < N. Radix select over the 32-bit keys
 // (8 bits per level, one LDS histogram, the chosen bin broadcast by readlane), so every level costs
 // one pass of 32-bit arithmetic and two wave syncs; entries that share the K-th key are resolved
-// by the 64-bit select over that key's entries only. hist: >= 256 + 16 uint32 of LDS.
+// by the 64-bit select over that key's entries only. hist: >= 4 * 256 uint32 of LDS.
 template <int U>
 __device__ uint64_t wave_kth_regs(const uint64_t (&v)[U], int N, int K, uint32_t* hist) {
   const int lane = threadIdx.x & 63;
@@ -1216,8 +1216,11 @@ __device__ uint64_t wave_kth_regs(const uint64_t (&v)[U], int N, int K, uint32_t
       }
     return K >= N ? wave_max_u64(emx) : wave_min_u64(emn);
   }
-  *(u32x4*)(hist + 4 * lane) = (u32x4){0u, 0u, 0u, 0u};
+  // four sub-histograms (lane & 3) cut the same-bin conflicts of the LDS atomics by four
+#pragma unroll
+  for (int g = 0; g < 4; ++g) *(u32x4*)(hist + 256 * g + 4 * lane) = (u32x4){0u, 0u, 0u, 0u};
   wave_sync();
+  uint32_t* hsub = hist + 256 * (lane & 3);
   uint32_t base = kmn;
   int below = 0;
   const uint32_t span = kmx - kmn;
@@ -1229,11 +1232,15 @@ __device__ uint64_t wave_kth_regs(const uint64_t (&v)[U], int N, int K, uint32_t
       if (lane + 64 * u < N) {
         const uint32_t key = (uint32_t)(v[u] >> 32);
         const uint32_t r = (key - base) >> sh;
-        if (key >= base && r < 256u) atomicAdd(&hist[r], 1u);
+        if (key >= base && r < 256u) atomicAdd(&hsub[r], 1u);
       }
     wave_sync();
-    const u32x4 c = *(const u32x4*)(hist + 4 * lane);
-    *(u32x4*)(hist + 4 * lane) = (u32x4){0u, 0u, 0u, 0u};
+    u32x4 c = (u32x4){0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      c += *(const u32x4*)(hist + 256 * g + 4 * lane);
+      *(u32x4*)(hist + 256 * g + 4 * lane) = (u32x4){0u, 0u, 0u, 0u};
+    }
     wave_sync();
     const uint32_t tot = c[0] + c[1] + c[2] + c[3];
     const uint32_t incl = wave_incl_scan_u32(tot, lane);
@@ -1284,7 +1291,7 @@ __device__ uint64_t wave_kth_regs(const uint64_t (&v)[U], int N, int K, uint32_t
 __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
   __shared__ int off_all[4][MERGE_MAX_SRC + 1];
   __shared__ uint16_t map_all[4][64 * MR_U];
-  __shared__ __attribute__((aligned(16))) uint32_t kth_all[4][256 + 32];
+  __shared__ __attribute__((aligned(16))) uint32_t kth_all[4][4 * 256 + 32];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = blockIdx.x * 4 + wv;
   if (q >= (a.n_dev ? *a.n_dev : a.n)) return;   // wave-uniform; only wave-level sync below
