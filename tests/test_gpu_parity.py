@@ -277,3 +277,26 @@ def test_metamorphic_invariants(gpu, knns_mod, oracle_mod, prec):
     Df, If = oracle_mod.exact_topk(Q[:10], Rs, 200)
     assert np.array_equal(I4, If.astype(np.int64)) and np.array_equal(D4, Df.astype(np.float32))
     assert all(sorted(r.tolist()) == list(range(200)) for r in I4)
+
+
+def test_large_corpus_row_offsets(gpu, knns_mod, oracle_mod):
+    """A corpus of 20,000,003 rows (not a multiple of the 128-row stage): row indices above 2^24
+    come back exactly (int32 through the native API) and the scans address stages whose byte
+    offsets pass 2 GiB (FP32 operand rows: 2.9 GB). Integer data (exact in BF16) makes the BF16
+    answer the exact one too; each query's nearest rows are planted above row 2^24."""
+    m, d, n, k = 20_000_003, 32, 8, 10
+    _print_case("large_corpus", m=m, d=d, n=n, k=k)
+    g = np.random.default_rng(29)
+    R = g.integers(0, 16, (m, d), dtype=np.int8).astype(np.float32)
+    Q = g.integers(0, 16, (n, d), dtype=np.int8).astype(np.float32)
+    for i in range(n):
+        R[(1 << 24) + 1 + 7 * i] = Q[i]            # distance 0
+        R[m - 1 - i] = Q[i]
+        R[m - 1 - i, i] += 1.0                      # distance 1
+    D, I = oracle_mod.exact_topk(Q, R, k)
+    assert np.all(I[:, 0] > (1 << 24)) and np.all(I[:, 1] >= m - n)
+    with knns_mod.Corpus(R, bf16=True) as c:
+        for prec in ("fp32", "bf16"):
+            dd, ii = c.search(Q, k, precision=prec)
+            assert np.array_equal(ii.astype(np.int64), I.astype(np.int64)), prec
+            assert np.array_equal(dd, D.astype(np.float32)), prec
