@@ -81,13 +81,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// s_waitcnt vmcnt(N) leaving expcnt/lgkmcnt unconstrained (gfx9 encoding)
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-  static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
-  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
-
 template <bool WAVE>
 __device__ __forceinline__ void group_sync() {
   if constexpr (WAVE) wave_sync(); else __syncthreads();
@@ -953,23 +946,6 @@ __global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
 constexpr int MERGE_MAX_SRC = 1 + 2 * 128;
 constexpr int MW_ENT = 1024;   // entries gathered to LDS per wave (8 KB)
 
-// wave-level bitonic sort of P (power of two <= MW_ENT) u64 entries in LDS, ascending
-__device__ __forceinline__ void wave_bitonic_u64(uint64_t* v, int P) {
-  const int lane = threadIdx.x & 63;
-  for (int size = 2; size <= P; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      wave_sync();
-      for (int t = lane; t < P / 2; t += 64) {
-        const int pos = 2 * t - (t & (stride - 1));
-        const bool up = ((pos & size) == 0);
-        const uint64_t x = v[pos], y = v[pos + stride];
-        if ((x > y) == up) { v[pos] = y; v[pos + stride] = x; }
-      }
-    }
-  }
-  wave_sync();
-}
-
 __device__ __forceinline__ void ce_u64(uint64_t& x, uint64_t& y, bool up) {
   const bool sw = (x > y) == up;
   const uint64_t lo = sw ? y : x;
@@ -977,11 +953,12 @@ __device__ __forceinline__ void ce_u64(uint64_t& x, uint64_t& y, bool up) {
   x = lo;
 }
 
-// Ascending bitonic sort of P (a power of two, 64 <= P <= 1024) u64 entries in LDS by one wave: the same network as
-// wave_bitonic_u64, with every stage of stride < 8 run in registers on 8-entry blocks (block b =
-// entries 8b .. 8b+7, read and written as 16-byte units) -- one block pass per level instead of
-// three LDS stages, so a 1024-entry sort makes 28 LDS stages and 8 block passes instead of 55
-// LDS stages.
+// Ascending bitonic sort of P (a power of two, 64 <= P <= 1024) u64 entries in LDS by one wave.
+// Stage (size, stride) compares entries pos and pos + stride (pos with bit `stride` clear) and
+// orders them ascending when pos & size == 0, descending otherwise. Stages of stride >= 8 run
+// through LDS; every stage of stride < 8 runs in registers on 8-entry blocks (block b = entries
+// 8b .. 8b+7, read and written as 16-byte units) -- one block pass per level instead of three
+// LDS stages, so a 1024-entry sort makes 28 LDS stages and 8 block passes instead of 55.
 __device__ void wave_bitonic_u64_r8(uint64_t* v, int P) {
   typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
   const int lane = threadIdx.x & 63;
