@@ -58,6 +58,15 @@ double gpuknnsBitonic(knntype* query, knntype* data, knntype* values, knntype* i
 double gpuknnsHeap(knntype* query, knntype* data, knntype* values, knntype* indices,
                    int objects, int numQueries, int attributes, int k, int numStreams);
 
+/* cuknns.h:79 (src/cuknnBitonic.cu:621-847, unfinished there: its search call is commented
+ * out). Bucketed search: the corpus is c consecutive buckets of bucketSize[j] rows; the queries
+ * query_offsets[j] .. +query_sizes[j] get their exact top-k within bucket j, at their own rows of
+ * values / indices (k leading), indices = corpus rows; slots past a bucket smaller than k hold
+ * +inf and -1. dp and numStreams are ignored. Seconds, FLT_MAX on failure. */
+float gpuknnLSH(knntype* query, knntype* data, knntype* values, knntype* indices, knntype* dp,
+                int objects, int numQueries, int attributes, int k, int numStreams,
+                int* bucketSize, int* query_offsets, int c, int* query_sizes);
+
 /* cuknns.h:85. Table-driven: no trial runs on random data (src/knnsplan.cu:43-57 ran 80). */
 void knnsplan(knnplan* plan, long int N, long int Q, long int D, long int k);
 

@@ -80,6 +80,102 @@ double run(knntype* query, knntype* data, knntype* values, knntype* indices, int
   return (double)ms * 1e-3;
 }
 
+// Bucketed search behind gpuknnLSH. Bucket j is corpus rows [b_j, b_j + bucketSize[j]) with
+// b_j = bucketSize[0] + ... + bucketSize[j-1]; cluster j is query rows
+// [query_offsets[j], query_offsets[j] + query_sizes[j]). Each bucket with queries becomes a
+// resident corpus of its own and its cluster one search over it.
+double run_buckets(knntype* query, knntype* data, knntype* values, knntype* indices, int objects,
+                   int numQueries, int attributes, int k, const int* bucketSize,
+                   const int* query_offsets, int c, const int* query_sizes) {
+  g_err.clear();
+  g_times = {0.0, 0.0, 0.0};
+  if (!query || !data || !values || !indices || !bucketSize || !query_offsets || !query_sizes) {
+    g_err = "null pointer argument";
+    return FLT_MAX;
+  }
+  if (objects < 1 || numQueries < 0 || attributes < 1 || k < 1 || k > 1024 || c < 1) {
+    char b[192];
+    snprintf(b, sizeof(b), "invalid sizes: objects=%d numQueries=%d attributes=%d k=%d clusters=%d",
+             objects, numQueries, attributes, k, c);
+    g_err = b;
+    return FLT_MAX;
+  }
+  std::vector<long> begin((size_t)c);
+  long rows = 0;
+  for (int j = 0; j < c; ++j) {
+    char b[192];
+    if (bucketSize[j] < 0 || query_sizes[j] < 0 || query_offsets[j] < 0 ||
+        (long)query_offsets[j] + query_sizes[j] > numQueries) {
+      snprintf(b, sizeof(b), "cluster %d: bucket size %d, queries [%d, +%d) of %d", j,
+               bucketSize[j], query_offsets[j], query_sizes[j], numQueries);
+      g_err = b;
+      return FLT_MAX;
+    }
+    if (bucketSize[j] == 0 && query_sizes[j] > 0) {
+      snprintf(b, sizeof(b), "cluster %d has %d queries and an empty bucket", j, query_sizes[j]);
+      g_err = b;
+      return FLT_MAX;
+    }
+    begin[(size_t)j] = rows;
+    rows += bucketSize[j];
+  }
+  if (rows > objects) {
+    char b[128];
+    snprintf(b, sizeof(b), "buckets hold %ld rows, the corpus %d", rows, objects);
+    g_err = b;
+    return FLT_MAX;
+  }
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) {
+    g_err = "hipEventCreate failed (no device?)";
+    (void)hipGetLastError();
+    return FLT_MAX;
+  }
+  (void)hipEventRecord(t0, nullptr);
+  knns_status st = KNNS_OK;
+  std::vector<int32_t> idx;
+  double scan_s = 0.0, select_s = 0.0;
+  for (int j = 0; j < c && st == KNNS_OK; ++j) {
+    const long nq = query_sizes[j];
+    if (nq == 0) continue;
+    const long q0 = query_offsets[j];
+    knns_corpus cj = nullptr;
+    st = knns_corpus_create(&cj, data + begin[(size_t)j] * attributes, kDtype, bucketSize[j],
+                            attributes, 0);
+    if (st == KNNS_OK) {
+      knns_params p;
+      knns_params_init(&p, kPrec, k);
+      p.select = KNNS_SELECT_THRESHOLD;
+      p.flags = KNNS_KERNEL_TIMES;
+      idx.resize((size_t)nq * k);
+      st = knns_search(cj, query + q0 * attributes, kDtype, nq, &p, values + q0 * k, idx.data());
+      if (st == KNNS_OK) {
+        knns_stats stats{};
+        knns_last_stats(&stats);
+        scan_s += stats.scan_s;
+        select_s += stats.select_s;
+        // bucket-relative rows -> corpus rows; -1 (slots past a bucket smaller than k) stays
+        for (size_t i = 0; i < idx.size(); ++i)
+          indices[q0 * k + (long)i] =
+              (knntype)(idx[i] < 0 ? -1L : (long)idx[i] + begin[(size_t)j]);
+      }
+    }
+    if (st != KNNS_OK) g_err = knns_last_error();
+    knns_corpus_destroy(cj);
+  }
+  (void)hipEventRecord(t1, nullptr);
+  (void)hipEventSynchronize(t1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, t0, t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (st != KNNS_OK) return FLT_MAX;
+  g_times.dst_time = scan_s * 1e3;
+  g_times.srch_time = select_s * 1e3;
+  g_times.knn_time = (double)ms;
+  return (double)ms * 1e-3;
+}
+
 }  // namespace
 
 extern "C" {
@@ -95,6 +191,26 @@ double gpuknnsHeap(knntype* query, knntype* data, knntype* values, knntype* indi
   (void)numStreams;
   return run(query, data, values, indices, objects, numQueries, attributes, k,
              KNNS_SELECT_THRESHOLD);
+}
+
+// gpuknnLSH (include/cuknns.h:79; src/cuknnBitonic.cu:621-847). The reference's version is
+// unfinished: its search call is commented out (cuknnBitonic.cu:772), so it only copies buckets
+// and returns unwritten output buffers. This implements what its arguments describe: the corpus
+// is c consecutive buckets (bucketSize[j] rows each, from row 0 in order) and the queries of
+// cluster j, rows query_offsets[j] .. +query_sizes[j], get their exact top-k within bucket j,
+// written at their own rows of values / indices (k leading, like gpuknnsBitonic), indices being
+// corpus rows (0-based). Slots past a bucket smaller than k hold distance +inf and index -1.
+// dp (the reference's precomputed ||r||^2) is not needed and is ignored; bucket j starts at the
+// sum of the sizes before it (the reference's `offset & ((objects>>1)-1)`, cuknnBitonic.cu:755,
+// wraps for corpora that are not powers of two). Returns seconds, FLT_MAX on failure.
+float gpuknnLSH(knntype* query, knntype* data, knntype* values, knntype* indices, knntype* dp,
+                int objects, int numQueries, int attributes, int k, int numStreams,
+                int* bucketSize, int* query_offsets, int c, int* query_sizes) {
+  (void)dp;
+  (void)numStreams;
+  const double t = run_buckets(query, data, values, indices, objects, numQueries, attributes, k,
+                               bucketSize, query_offsets, c, query_sizes);
+  return t >= (double)FLT_MAX ? FLT_MAX : (float)t;
 }
 
 // src/knnsplan.cu:25-105 timed both algorithms over streams {1,2,4,8} x 10 iterations on random

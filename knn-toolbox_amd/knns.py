@@ -3,7 +3,7 @@
 Two layers, mirroring the C headers:
 
 * ``dropin`` / ``DropIn`` -- the reference's C API (include/cuknns.h): ``gpuknnsBitonic``,
-  ``gpuknnsHeap``, ``knnsplan``, ``knnsexecute``, with knntype (float32, or float64 for the
+  ``gpuknnsHeap``, ``gpuknnLSH``, ``knnsplan``, ``knnsexecute``, with knntype (float32, or float64 for the
   double library) row-major inputs and Q x k outputs holding indices as knntype values, exactly as
   the reference's C callers see them (src/knnTest.cu:93-100).
 * ``Corpus`` -- the native engine (include/knns_native.h): resident corpus, int32 indices,
@@ -144,6 +144,9 @@ def dropin(double: bool = False):
             f = getattr(L, fn)
             f.argtypes = [kt, kt, kt, kt, I, I, I, I, I]
             f.restype = ctypes.c_double
+        ip = ctypes.c_void_p
+        L.gpuknnLSH.argtypes = [kt, kt, kt, kt, kt, I, I, I, I, I, ip, ip, I, ip]
+        L.gpuknnLSH.restype = ctypes.c_float
         L.knnsGetLastError.argtypes = []
         L.knnsGetLastError.restype = ctypes.c_char_p
         L.knnsGetLastTimes.argtypes = []
@@ -310,6 +313,32 @@ def gpuknns_bitonic(query, data, k: int, double: bool = False, num_streams: int 
 
 def gpuknns_heap(query, data, k: int, double: bool = False, num_streams: int = 1):
     return _dropin_call("gpuknnsHeap", query, data, k, double, num_streams)
+
+
+def gpuknn_lsh(query, data, k: int, bucket_sizes, query_offsets, query_sizes, double: bool = False,
+               num_streams: int = 1):
+    """gpuknnLSH(query, data, values, indices, dp, objects, numQueries, attributes, k, numStreams,
+    bucketSize, query_offsets, c, query_sizes): the queries of cluster j (rows query_offsets[j] ..
+    +query_sizes[j]) get their exact top-k within bucket j (the j-th run of bucket_sizes[j]
+    consecutive corpus rows). Returns (values (nq, k), indices (nq, k) as knntype corpus rows,
+    seconds); rows of queries in no cluster are left as zeros."""
+    L = dropin(double)
+    kt = np.float64 if double else np.float32
+    q = np.ascontiguousarray(query, dtype=kt)
+    x = np.ascontiguousarray(data, dtype=kt)
+    bs = np.ascontiguousarray(bucket_sizes, dtype=np.int32)
+    qo = np.ascontiguousarray(query_offsets, dtype=np.int32)
+    qs = np.ascontiguousarray(query_sizes, dtype=np.int32)
+    if not (bs.shape == qo.shape == qs.shape) or bs.ndim != 1:
+        raise KnnsError(KNNS_ERR_INVALID, "bucket_sizes, query_offsets and query_sizes differ in length")
+    nq, d = q.shape
+    vals = np.zeros((nq, k), dtype=kt)
+    inds = np.zeros((nq, k), dtype=kt)
+    t = L.gpuknnLSH(q.ctypes.data, x.ctypes.data, vals.ctypes.data, inds.ctypes.data, None, x.shape[0], nq, d,
+                    k, num_streams, bs.ctypes.data, qo.ctypes.data, len(bs), qs.ctypes.data)
+    if t >= float(np.finfo(np.float32).max):
+        raise KnnsError(KNNS_ERR_DEVICE, L.knnsGetLastError().decode(errors="replace"))
+    return vals, inds, t
 
 
 # ---- raw .bin files of the knnTest driver (src/knnTest.cu:17-48, demo/exportData.m,

@@ -12,6 +12,8 @@ Contents
       brute force and the key formulation the GPU code ranks on, include/utils.h:957-985).
     - ``exact_topk``: FP64 brute force with the canonical distance (sequential FMA of squared
       differences) and (dist, index) ordering.
+* ``bucketed_topk``     -- the bucketed search gpuknnLSH's arguments describe (include/cuknns.h:79):
+  ``exact_topk`` per (bucket, query cluster) pair, indices shifted to corpus rows.
 * Data generators, re-exported from ``knn-toolbox_amd/workloads.py`` (the synthetic inputs of every
   workload, written down once there).
 * Python restatements of the reference's selection kernels, run on explicit key matrices:
@@ -103,6 +105,29 @@ def exact_topk(Q: np.ndarray, R: np.ndarray, k: int, self_ids: np.ndarray | None
         self_ids = np.ascontiguousarray(self_ids, dtype=np.int64)
         sp = _p(self_ids)
     lib.oracle_exact_topk(_p(Q), _p(R), 0 if Q.dtype == np.float32 else 1, n, m, d, k, sp, _p(D), _p(I))
+    return D, I
+
+
+def bucketed_topk(Q: np.ndarray, R: np.ndarray, k: int, bucket_sizes, query_offsets, query_sizes):
+    """What gpuknnLSH's arguments describe (include/cuknns.h:79, src/cuknnBitonic.cu:621-847, whose
+    search call is commented out at :772): bucket j = the j-th run of bucket_sizes[j] consecutive
+    corpus rows; the queries query_offsets[j] .. +query_sizes[j] get ``exact_topk`` over bucket j
+    only, with corpus-row indices. Slots past a bucket smaller than k are (+inf, -1); rows of
+    queries in no cluster stay (0, 0). Returns (dist f64 (n,k), idx int64 (n,k))."""
+    n = Q.shape[0]
+    D = np.zeros((n, k), dtype=np.float64)
+    I = np.zeros((n, k), dtype=np.int64)
+    b = 0
+    for sz, q0, nq in zip(bucket_sizes, query_offsets, query_sizes):
+        sz, q0, nq = int(sz), int(q0), int(nq)
+        if nq > 0:
+            kk = min(k, sz)
+            Dj, Ij = exact_topk(Q[q0:q0 + nq], R[b:b + sz], kk)
+            D[q0:q0 + nq, :kk] = Dj
+            I[q0:q0 + nq, :kk] = Ij.astype(np.int64) + b
+            D[q0:q0 + nq, kk:] = np.inf
+            I[q0:q0 + nq, kk:] = -1
+        b += sz
     return D, I
 
 

@@ -115,3 +115,27 @@ def test_dropin_rejects_bad_sizes_without_gpu(knns_mod):
     t = L.gpuknnsHeap(q.ctypes.data, q.ctypes.data, out.ctypes.data, out.ctypes.data, 0, 2, 4, 4, 1)
     assert t >= float(np.finfo(np.float32).max)
     assert b"invalid sizes" in L.knnsGetLastError()
+
+
+def test_bucketed_search_rejects_bad_arguments_without_gpu(knns_mod):
+    import numpy as np
+    L = knns_mod.dropin(False)
+    q = np.zeros((6, 4), np.float32)
+    x = np.zeros((10, 4), np.float32)
+    out = np.zeros((6, 3), np.float32)
+
+    def call(bs, qo, qs, objects=10, nq=6, c=None):
+        bs, qo, qs = (np.asarray(a, np.int32) for a in (bs, qo, qs))
+        return L.gpuknnLSH(q.ctypes.data, x.ctypes.data, out.ctypes.data, out.ctypes.data, None, objects, nq, 4,
+                           3, 1, bs.ctypes.data, qo.ctypes.data, len(bs) if c is None else c, qs.ctypes.data)
+
+    flt_max = float(np.finfo(np.float32).max)
+    assert call([5, 6], [0, 3], [3, 3]) >= flt_max                 # buckets hold 11 rows of 10
+    assert b"buckets hold 11 rows" in L.knnsGetLastError()
+    assert call([5, 5], [0, 4], [3, 3]) >= flt_max                 # cluster 1's queries run past 6
+    assert b"cluster 1" in L.knnsGetLastError()
+    assert call([5, 0], [0, 3], [3, 3]) >= flt_max                 # queries with an empty bucket
+    assert b"empty bucket" in L.knnsGetLastError()
+    assert call([5, -1], [0, 3], [3, 0]) >= flt_max
+    assert call([5, 5], [0, 3], [3, 3], c=0) >= flt_max
+    assert b"invalid sizes" in L.knnsGetLastError()

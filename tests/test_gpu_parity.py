@@ -89,6 +89,29 @@ def test_shapes_match_oracle(gpu, knns_mod, oracle_mod, m, n, d, k):
         assert np.all(inds[:, m:] == -1)
 
 
+@pytest.mark.parametrize("double", [False, True])
+def test_bucketed_search_matches_oracle(gpu, knns_mod, oracle_mod, double):
+    """gpuknnLSH (include/cuknns.h:79): each query cluster's exact top-k within its bucket, with
+    corpus-row indices, equal to oracle.bucketed_topk -- indices identical and distances equal to
+    the FP64 oracle's rounded to knntype (the gpuknnsHeap contract). Covers a bucket smaller than
+    k (+inf / -1 slots), an empty bucket, a cluster without queries, clusters whose query rows are
+    out of cluster order, query rows in no cluster (left untouched) and corpus rows past the last
+    bucket."""
+    dt = np.float64 if double else np.float32
+    R = oracle_mod.uniform(20000, 64, 41, dtype=dt)
+    Q = oracle_mod.uniform(700, 64, 42, dtype=dt)
+    bs, qo, qs = [7000, 5, 0, 9000, 3000], [300, 0, 250, 250, 600], [300, 200, 0, 50, 100]
+    k = 32
+    _print_case("bucketed", double=double, m=20000, n=700, d=64, k=k, clusters=len(bs))
+    D, I = oracle_mod.bucketed_topk(Q, R, k, bs, qo, qs)
+    vals, inds, t = knns_mod.gpuknn_lsh(Q, R, k, bs, qo, qs, double=double)
+    assert t > 0 and vals.dtype == dt
+    assert np.array_equal(inds.astype(np.int64), I)
+    assert np.array_equal(vals, D.astype(dt))
+    assert np.all(inds[0:200, 5:] == -1) and np.all(np.isinf(vals[0:200, 5:]))
+    assert np.all(vals[200:250] == 0)
+
+
 def test_ties_go_to_lower_index(gpu, knns_mod, oracle_mod):
     rng = np.random.default_rng(3)
     base = rng.integers(-4, 5, size=(700, 12)).astype(np.float32)

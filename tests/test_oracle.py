@@ -228,3 +228,45 @@ def test_exact_topk_and_knnm_property_tiny_integer_inputs(oracle_mod):
             assert np.array_equal(Im, Ip) and np.array_equal(Dm.astype(np.float64), Dp)
 
     check()
+
+
+def _bucket_brute(Q, R, k, bucket_sizes, query_offsets, query_sizes):
+    """Independent restatement for bucketed_topk: exact integer distances to EVERY corpus row, rows
+    outside the query's bucket masked out by a row-id range test, then a stable sort over the whole
+    corpus (so the indices are corpus rows without any per-bucket offset arithmetic)."""
+    n, m = Q.shape[0], R.shape[0]
+    D = np.zeros((n, k))
+    I = np.zeros((n, k), dtype=np.int64)
+    d2 = ((Q.astype(np.int64)[:, None, :] - R.astype(np.int64)[None, :, :]) ** 2).sum(-1)
+    ends = np.cumsum(bucket_sizes)
+    rows = np.arange(m)
+    for j, (q0, nq) in enumerate(zip(query_offsets, query_sizes)):
+        lo, hi = ends[j] - bucket_sizes[j], ends[j]
+        inb = (rows >= lo) & (rows < hi)
+        for i in range(q0, q0 + nq):
+            key = np.where(inb, d2[i], np.iinfo(np.int64).max)
+            order = np.lexsort((rows, key))[:k]
+            ok = inb[order]
+            D[i] = np.where(ok, key[order].astype(np.float64), np.inf)
+            I[i] = np.where(ok, order, -1)
+    return D, I
+
+
+def test_bucketed_topk_pinned(oracle_mod):
+    rng = np.random.default_rng(17)
+    R = _int_data(rng, 230, 6)
+    R[100:110] = R[3]                     # ties inside bucket 1 and across buckets 0 / 1
+    Q = _int_data(rng, 40, 6)
+    k = 9
+    # one bucket holding the corpus, one cluster holding every query: exact_topk itself
+    D1, I1 = oracle_mod.bucketed_topk(Q, R, k, [230], [0], [40])
+    De, Ie = oracle_mod.exact_topk(Q, R, k)
+    assert np.array_equal(I1, Ie) and np.array_equal(D1, De)
+    # uneven buckets, a bucket smaller than k, an empty bucket without queries, a cluster without
+    # queries, clusters whose query rows are not in cluster order, rows past the last bucket
+    bs, qo, qs = [60, 90, 5, 0, 70], [30, 0, 12, 40, 20], [10, 12, 8, 0, 10]
+    D, I = oracle_mod.bucketed_topk(Q, R, k, bs, qo, qs)
+    Db, Ib = _bucket_brute(Q, R, k, bs, qo, qs)
+    assert np.array_equal(I, Ib) and np.array_equal(D, Db)
+    assert np.all(I[12:20, 5:] == -1) and np.all(I[12:20, :5] >= 150) and np.all(I[12:20, :5] < 155)
+    assert np.all((I[20:30] >= 155) & (I[20:30] < 225))
