@@ -94,6 +94,12 @@ knns_status knns_search(knns_corpus c, const void* queries, int dtype, long n,
 knns_status knns_search_multi(knns_corpus c, const void* queries, int dtype, long n,
                               const int* kq, const knns_params* p, void* dist, int32_t* idx);
 
+/* knns_search restricted to corpus rows [row_begin, row_end) (0 <= row_begin < row_end <= rows);
+ * indices are corpus rows. No counterpart in the reference's exported API; it is the search of
+ * one bucket of gpuknnLSH (cuknns.h:79) over a corpus uploaded once for all buckets. */
+knns_status knns_search_rows(knns_corpus c, long row_begin, long row_end, const void* queries,
+                             int dtype, long n, const knns_params* p, void* dist, int32_t* idx);
+
 /* Planner view: the launch configuration knns_search would use (no GPU work). */
 knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns_stats* out);
 

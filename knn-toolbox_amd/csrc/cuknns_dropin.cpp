@@ -82,8 +82,8 @@ double run(knntype* query, knntype* data, knntype* values, knntype* indices, int
 
 // Bucketed search behind gpuknnLSH. Bucket j is corpus rows [b_j, b_j + bucketSize[j]) with
 // b_j = bucketSize[0] + ... + bucketSize[j-1]; cluster j is query rows
-// [query_offsets[j], query_offsets[j] + query_sizes[j]). Each bucket with queries becomes a
-// resident corpus of its own and its cluster one search over it.
+// [query_offsets[j], query_offsets[j] + query_sizes[j]). The buckets' rows are uploaded once as
+// one resident corpus and each cluster is one search over its bucket's row range.
 double run_buckets(knntype* query, knntype* data, knntype* values, knntype* indices, int objects,
                    int numQueries, int attributes, int k, const int* bucketSize,
                    const int* query_offsets, int c, const int* query_sizes) {
@@ -125,6 +125,9 @@ double run_buckets(knntype* query, knntype* data, knntype* values, knntype* indi
     g_err = b;
     return FLT_MAX;
   }
+  long queries = 0;
+  for (int j = 0; j < c; ++j) queries += query_sizes[j];
+  if (queries == 0) return 0.0;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) {
     g_err = "hipEventCreate failed (no device?)";
@@ -132,37 +135,32 @@ double run_buckets(knntype* query, knntype* data, knntype* values, knntype* indi
     return FLT_MAX;
   }
   (void)hipEventRecord(t0, nullptr);
-  knns_status st = KNNS_OK;
+  knns_corpus cp = nullptr;
+  knns_status st = knns_corpus_create(&cp, data, kDtype, rows, attributes, 0);
   std::vector<int32_t> idx;
   double scan_s = 0.0, select_s = 0.0;
   for (int j = 0; j < c && st == KNNS_OK; ++j) {
     const long nq = query_sizes[j];
     if (nq == 0) continue;
     const long q0 = query_offsets[j];
-    knns_corpus cj = nullptr;
-    st = knns_corpus_create(&cj, data + begin[(size_t)j] * attributes, kDtype, bucketSize[j],
-                            attributes, 0);
+    knns_params p;
+    knns_params_init(&p, kPrec, k);
+    p.select = KNNS_SELECT_THRESHOLD;
+    p.flags = KNNS_KERNEL_TIMES;
+    idx.resize((size_t)nq * k);
+    st = knns_search_rows(cp, begin[(size_t)j], begin[(size_t)j] + bucketSize[j],
+                          query + q0 * attributes, kDtype, nq, &p, values + q0 * k, idx.data());
     if (st == KNNS_OK) {
-      knns_params p;
-      knns_params_init(&p, kPrec, k);
-      p.select = KNNS_SELECT_THRESHOLD;
-      p.flags = KNNS_KERNEL_TIMES;
-      idx.resize((size_t)nq * k);
-      st = knns_search(cj, query + q0 * attributes, kDtype, nq, &p, values + q0 * k, idx.data());
-      if (st == KNNS_OK) {
-        knns_stats stats{};
-        knns_last_stats(&stats);
-        scan_s += stats.scan_s;
-        select_s += stats.select_s;
-        // bucket-relative rows -> corpus rows; -1 (slots past a bucket smaller than k) stays
-        for (size_t i = 0; i < idx.size(); ++i)
-          indices[q0 * k + (long)i] =
-              (knntype)(idx[i] < 0 ? -1L : (long)idx[i] + begin[(size_t)j]);
-      }
+      knns_stats stats{};
+      knns_last_stats(&stats);
+      scan_s += stats.scan_s;
+      select_s += stats.select_s;
+      // corpus rows; -1 in slots past a bucket smaller than k
+      for (size_t i = 0; i < idx.size(); ++i) indices[q0 * k + (long)i] = (knntype)idx[i];
     }
-    if (st != KNNS_OK) g_err = knns_last_error();
-    knns_corpus_destroy(cj);
   }
+  if (st != KNNS_OK) g_err = knns_last_error();
+  knns_corpus_destroy(cp);
   (void)hipEventRecord(t1, nullptr);
   (void)hipEventSynchronize(t1);
   float ms = 0.f;

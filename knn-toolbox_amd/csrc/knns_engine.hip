@@ -459,8 +459,10 @@ bool is_device_ptr(const void* p) {
   return at.type == hipMemoryTypeDevice;
 }
 
+// Searches corpus rows [row_begin, row_end) (row_end < 0: to the last row); indices are corpus rows.
 knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, const int* kq_host,
-                      int kuni, const knns_params* p, void* dist, int32_t* idx) {
+                      int kuni, const knns_params* p, void* dist, int32_t* idx,
+                      long row_begin = 0, long row_end = -1) {
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lock(E.mu);
   g_err.clear();
@@ -487,7 +489,11 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   const bool out64 = c->dtype == KNNS_F64 || (p->flags & KNNS_DIST_F64);
   const bool ktimes = (p->flags & KNNS_KERNEL_TIMES) != 0;
   hipStream_t st = p->stream ? (hipStream_t)p->stream : E.stream;
-  const long m = c->m;
+  if (row_end < 0) row_end = c->m;
+  if (row_begin < 0 || row_begin >= row_end || row_end > c->m)
+    return fail(KNNS_ERR_INVALID, "row range [%ld, %ld) outside the corpus' %ld rows", row_begin, row_end,
+                c->m);
+  const long m = row_end - row_begin;   // rows searched
 
   // per-query k and shortlist
   const bool uniform = kq_host == nullptr;
@@ -525,6 +531,10 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   const Tuning tu = tuning_for(prec, c->d, kmax, n);
   Plan P = make_plan(m, c->d, n, kmax, cmax, prec, E.cus, p->ratio > 0 ? p->ratio : tu.ratio, p->slices,
                      tu.min_waves);
+  for (Round& r : P.rounds) {   // planned over the range's rows, from its first row
+    r.begin += row_begin;
+    r.end += row_begin;
+  }
   if (P.dp != c->dp) return fail(KNNS_ERR_INVALID, "corpus/plan dimension mismatch");
 
   // workspace (grow-only engine slots)
@@ -689,7 +699,9 @@ knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, lo
   c->dtype = dtype;
   c->flags = flags;
   dims_for(d, &c->dp, &c->dc);
-  c->rows_pad = round_up(m, 128);  // whole scan stages (64 or 128 rows) never leave the padding
+  // whole scan stages (64 or 128 rows) never leave the padding, also for a search over a row
+  // range (knns_search_rows) whose slices start at any row: 128 rows of slack past round_up(m, 128)
+  c->rows_pad = round_up(m, 128) + 128;
   c->ld32 = c->dp + 4;
   c->ld16 = c->dp + 16;
   if (flags & KNNS_CORPUS_FP8) {
@@ -716,7 +728,7 @@ knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, lo
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP64 master"));
   const size_t esz = dtype == KNNS_F64 ? 8 : 4;
   const bool dev = (flags & KNNS_CORPUS_DEVICE_PTR) != 0;
-  const long chunk = dev ? m : std::min<long>(m, 1L << 20);
+  const long chunk = dev ? c->rows_pad : std::min<long>(c->rows_pad, 1L << 20);
   void* stage = nullptr;
   if (!dev && hipMalloc(&stage, (size_t)chunk * d * esz) != hipSuccess)
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus staging buffer"));
@@ -792,6 +804,16 @@ knns_status knns_search_multi(knns_corpus c, const void* queries, int dtype, lon
                               const int* kq, const knns_params* p, void* dist, int32_t* idx) {
   if (!kq && n > 0) return fail(KNNS_ERR_INVALID, "null per-query k array");
   return do_search(c, queries, dtype, n, kq, 0, p, dist, idx);
+}
+
+knns_status knns_search_rows(knns_corpus c, long row_begin, long row_end, const void* queries,
+                             int dtype, long n, const knns_params* p, void* dist, int32_t* idx) {
+  if (!p) return fail(KNNS_ERR_INVALID, "null params");
+  if (!c) return fail(KNNS_ERR_INVALID, "null corpus");
+  if (row_begin < 0 || row_begin >= row_end || row_end > c->m)
+    return fail(KNNS_ERR_INVALID, "row range [%ld, %ld) outside the corpus' %ld rows", row_begin, row_end,
+                c->m);
+  return do_search(c, queries, dtype, n, nullptr, p->k, p, dist, idx, row_begin, row_end);
 }
 
 knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns_stats* out) {

@@ -1779,15 +1779,17 @@ bool scan_supported(int prec, int qb, int dc) {
   return dc == 32 || dc == 64 || dc == 96 || dc == 128;
 }
 
-// Row granularity of the stages: slice and round boundaries must be multiples of it, and the
-// corpus must be padded to it, so a stage never reads past the padded corpus.
+// Row granularity of the stages (64 or 128 rows): slice sizes are multiples of it. A slice may
+// start at any row; the corpus carries 128 rows past round_up(m, 128), so the last stage of a slice
+// (whose rows past row_end are masked) never reads past the allocation. Every row stride is a
+// multiple of 16 bytes, so the 16-byte DMA loads stay aligned from any starting row.
 int scan_stage_rows(int prec, int dp, int dc) {
   return ((prec == PREC_BF16 && dp == dc) || prec == PREC_FP8) ? 128 : 64;
 }
 
 hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, hipStream_t stream) {
   const int sr = scan_stage_rows(prec, a.dp, dc);
-  if ((a.row_begin % sr) || (a.slice_rows % sr)) return hipErrorInvalidValue;
+  if (a.row_begin < 0 || (a.slice_rows % sr)) return hipErrorInvalidValue;
   if (a.row_end > 0x7fffff80L) return hipErrorInvalidValue;   // int32 row indices
   if (prec == PREC_FP8) {
     // dc = dp = FP8 row bytes (64, 128, 256); ScanArgs strides are in bytes

@@ -104,6 +104,8 @@ def native():
         L.knns_search.restype = I
         L.knns_search_multi.argtypes = [V, V, I, L_, V, ctypes.POINTER(Params), V, V]
         L.knns_search_multi.restype = I
+        L.knns_search_rows.argtypes = [V, L_, L_, V, I, L_, ctypes.POINTER(Params), V, V]
+        L.knns_search_rows.restype = I
         L.knns_plan_describe.argtypes = [L_, I, L_, I, I, ctypes.POINTER(Stats)]
         L.knns_plan_describe.restype = I
         L.knns_last_error.argtypes = []
@@ -250,6 +252,23 @@ class Corpus:
                          KNNS_DIST_F64 if dist_f64 else 0)
         st = native().knns_search(self._h, q.ctypes.data, KNNS_F64 if self.dtype == np.float64 else KNNS_F32,
                                   n, ctypes.byref(p), D.ctypes.data, I.ctypes.data)
+        if st:
+            _err(st)
+        return D, I
+
+    def search_rows(self, queries, k: int, row_begin: int, row_end: int, precision: str = "fp32",
+                    select: str = "auto", shortlist: int = 0):
+        """search() over corpus rows [row_begin, row_end) only (knns_search_rows); indices are corpus
+        rows."""
+        q = np.ascontiguousarray(queries, dtype=self.dtype)
+        n = q.shape[0]
+        out64 = self.dtype == np.float64
+        D = np.empty((n, k), dtype=np.float64 if out64 else np.float32)
+        I = np.empty((n, k), dtype=np.int32)
+        p = self._params(precision, k, select, shortlist, False, 0, 0, 0, 0)
+        st = native().knns_search_rows(self._h, row_begin, row_end, q.ctypes.data,
+                                       KNNS_F64 if out64 else KNNS_F32, n, ctypes.byref(p), D.ctypes.data,
+                                       I.ctypes.data)
         if st:
             _err(st)
         return D, I

@@ -105,6 +105,11 @@ def test_native_argument_errors_without_gpu(knns_mod):
     assert L.knns_corpus_create(ctypes.byref(h), None, 0, 4, 4, 0) == knns_mod.KNNS_ERR_INVALID
     s = knns_mod.Stats()
     assert L.knns_plan_describe(10, 4, 1, 2000, 0, ctypes.byref(s)) == knns_mod.KNNS_ERR_INVALID
+    p = knns_mod.Params()
+    L.knns_params_init(ctypes.byref(p), 0, 4)
+    assert L.knns_search_rows(None, 0, 4, x.ctypes.data, 0, 4, ctypes.byref(p), x.ctypes.data,
+                              x.ctypes.data) == knns_mod.KNNS_ERR_INVALID
+    assert b"null corpus" in L.knns_last_error()
 
 
 def test_dropin_rejects_bad_sizes_without_gpu(knns_mod):

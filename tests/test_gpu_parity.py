@@ -112,6 +112,35 @@ def test_bucketed_search_matches_oracle(gpu, knns_mod, oracle_mod, double):
     assert np.all(vals[200:250] == 0)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8", "fp64"])
+def test_row_range_search_matches_oracle(gpu, knns_mod, oracle_mod, prec):
+    """knns_search_rows: the top-k over corpus rows [lo, hi) only, with corpus-row indices, equal to
+    the oracle over that slice, on integer data |x| <= 15 (exact keys in every format). Ranges start
+    off the scan's 64/128-row stage grid, one ends at the last corpus row (its last stage reads into
+    the corpus' slack rows), one holds a single row, one holds fewer rows than k (+inf / -1 slots),
+    one is the whole corpus, and duplicated rows straddle a range start (ties by lower index)."""
+    m, d, n, k = 9000, 40, 150, 24
+    g = np.random.default_rng(23)
+    dt = np.float64 if prec == "fp64" else np.float32
+    R = g.integers(-15, 16, (m, d)).astype(dt)
+    R[3000:3010] = R[2995]
+    Q = g.integers(-15, 16, (n, d)).astype(dt)
+    Q[:5] = R[2995]
+    _print_case("row_range", prec=prec, m=m, d=d, n=n, k=k)
+    with knns_mod.Corpus(R, bf16=prec == "bf16", fp8=prec == "fp8") as c:
+        for lo, hi in [(2997, 7421), (8833, 9000), (1, 9000), (4444, 4445), (77, 90), (0, 9000), (129, 1153)]:
+            kk = min(k, hi - lo)
+            D, I = oracle_mod.exact_topk(Q, R[lo:hi], kk)
+            dd, ii = c.search_rows(Q, k, lo, hi, precision=prec)
+            assert np.array_equal(ii[:, :kk], I.astype(np.int32) + lo), (lo, hi)
+            assert np.array_equal(dd[:, :kk], D.astype(dd.dtype)), (lo, hi)
+            assert np.all(ii[:, kk:] == -1) and np.all(np.isinf(dd[:, kk:]))
+        with pytest.raises(knns_mod.KnnsError, match="row range"):
+            c.search_rows(Q, k, 5, 9001, precision=prec)
+        with pytest.raises(knns_mod.KnnsError, match="row range"):
+            c.search_rows(Q, k, 10, 10, precision=prec)
+
+
 def test_ties_go_to_lower_index(gpu, knns_mod, oracle_mod):
     rng = np.random.default_rng(3)
     base = rng.integers(-4, 5, size=(700, 12)).astype(np.float32)
