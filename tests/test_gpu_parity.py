@@ -141,6 +141,28 @@ def test_row_range_search_matches_oracle(gpu, knns_mod, oracle_mod, prec):
             c.search_rows(Q, k, 10, 10, precision=prec)
 
 
+@pytest.mark.parametrize("double", [False, True])
+def test_corpus_upload_chunks(gpu, knns_mod, oracle_mod, double):
+    """A host corpus goes up in ~64 MB chunks through two alternating pinned halves. With 172-byte
+    (float, d=43) and 400-byte (double, d=50) rows the 400,000-row corpus spans 2 and 3 chunks whose
+    boundaries fall at rows 390,167 and 167,772 / 335,544: a query planted on a row of every chunk
+    (and next to each boundary) must find it, and sampled queries match the oracle exactly."""
+    dt = np.float64 if double else np.float32
+    m, d, k = 400_000, 50 if double else 43, 8
+    row_bytes = d * (8 if double else 4)
+    chunk = (64 << 20) // row_bytes
+    g = np.random.default_rng(31)
+    R = g.integers(-20, 21, (m, d)).astype(dt)
+    rows = sorted({1, chunk - 1, chunk, chunk + 1, min(2 * chunk + 5, m - 1), m - 1, 123_457})
+    Q = np.concatenate([R[rows] + dt(0.25), g.integers(-20, 21, (40, d)).astype(dt)])
+    _print_case("upload_chunks", double=double, m=m, d=d, chunk=chunk, planted=rows)
+    vals, inds, _ = knns_mod.gpuknns_heap(Q, R, k, double=double)
+    assert [int(inds[i, 0]) for i in range(len(rows))] == rows
+    D, I = oracle_mod.exact_topk(Q, R, k)
+    assert np.array_equal(inds.astype(np.int64), I.astype(np.int64))
+    assert np.array_equal(vals, D.astype(dt))
+
+
 def test_ties_go_to_lower_index(gpu, knns_mod, oracle_mod):
     rng = np.random.default_rng(3)
     base = rng.integers(-4, 5, size=(700, 12)).astype(np.float32)
