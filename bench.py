@@ -488,10 +488,15 @@ def main() -> int:
         for _ in range(3):
             corpus.search(Q, k, precision="bf16")
         host_io = {"bf16_host_buffers_qps": 3 * n / (time.perf_counter() - h0)}
-        h0 = time.perf_counter()
-        _, _, t_call = knns.gpuknns_heap(Q, R, k)
-        host_io["dropin_heap_call_qps"] = n / (time.perf_counter() - h0)
-        host_io["dropin_heap_call_s"] = t_call
+        knns.gpuknns_heap(Q, R, k)                 # first call: library load and set-up
+        walls, calls = [], []
+        for _ in range(3):
+            h0 = time.perf_counter()
+            _, _, t_call = knns.gpuknns_heap(Q, R, k)
+            walls.append(time.perf_counter() - h0)
+            calls.append(t_call)
+        host_io["dropin_heap_call_qps"] = n / float(np.median(walls))
+        host_io["dropin_heap_call_s"] = float(np.median(calls))
         log(f"host-memory boundary: {host_io}")
 
     # ---- serving: continuous batching over the resident corpus (knns_server_*) ----

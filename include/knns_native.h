@@ -100,6 +100,14 @@ knns_status knns_search_multi(knns_corpus c, const void* queries, int dtype, lon
 knns_status knns_search_rows(knns_corpus c, long row_begin, long row_end, const void* queries,
                              int dtype, long n, const knns_params* p, void* dist, int32_t* idx);
 
+/* knns_corpus_create + knns_search + knns_corpus_destroy in one call, for a corpus searched once
+ * (the whole-problem call of gpuknnsBitonic / gpuknnsHeap, src/cuknnBitonic.cu:422-618): host rows
+ * (m x d of `dtype`) upload on a second stream while the first threshold rounds, which scan a
+ * growing prefix of the corpus, already run. Host queries and outputs (KNNS_DEVICE_PTRS is
+ * rejected); results as knns_search. */
+knns_status knns_search_once(const void* data, int dtype, long m, int d, const void* queries,
+                             long n, const knns_params* p, void* dist, int32_t* idx);
+
 /* Planner view: the launch configuration knns_search would use (no GPU work). */
 knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns_stats* out);
 

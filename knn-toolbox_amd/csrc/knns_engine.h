@@ -87,6 +87,8 @@ struct SearchSizes {
 SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long total_out, bool excl,
                          bool out64);
 
+struct Upload;   // chunked corpus fill (knns_engine.hip)
+
 struct SearchLaunch {
   const knns_corpus_s* c = nullptr;
   const Plan* P = nullptr;
@@ -101,6 +103,7 @@ struct SearchLaunch {
   bool out64 = false;
   int bitonic = 0;
   const int* n_dev = nullptr;  // optional device query count <= n (graph-captured searches)
+  Upload* feed = nullptr;      // optional: corpus rows still uploading; each scan waits for its rows
 };
 
 // Timing hook: called with 0/1 around each scan launch, 2 after each merge, 3/4 around the final

@@ -55,6 +55,7 @@ struct Engine {
   int device = 0;
   int cus = 256;
   hipStream_t stream = nullptr;
+  hipStream_t ustream = nullptr;   // corpus uploads that overlap a search (knns_search_once)
   void* buf[W_NSLOTS] = {};
   size_t cap[W_NSLOTS] = {};
   std::vector<hipEvent_t> events;
@@ -70,6 +71,7 @@ struct Engine {
     KNNS_HIP(hipGetDeviceProperties(&prop, device));
     cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     KNNS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    KNNS_HIP(hipStreamCreateWithFlags(&ustream, hipStreamNonBlocking));
     ready = true;
     return KNNS_OK;
   }
@@ -353,6 +355,146 @@ SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long tot
   return z;
 }
 
+// memcpy split over up to 8 host threads (one thread moves about 30 GB/s on these hosts, the DMA
+// from pinned memory about 57 GB/s)
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t parts = std::min<size_t>(std::min(hw, 8u), bytes >> 22);   // >= 4 MB per thread
+  if (parts <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t step = ((bytes + parts - 1) / parts + 4095) & ~(size_t)4095;
+  std::vector<std::thread> th;
+  th.reserve(parts);
+  try {
+    for (size_t o = step; o < bytes; o += step)
+      th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, std::min(step, bytes - o)); });
+  } catch (...) {   // no thread: the caller's thread copies what is left
+    size_t o = step * (th.size() + 1);
+    if (o < bytes) std::memcpy((char*)dst + o, (const char*)src + o, bytes - o);
+  }
+  std::memcpy(dst, src, std::min(step, bytes));
+  for (auto& t : th) t.join();
+}
+
+// Chunked fill of a corpus' packed rows [0, rows_pad). Host data goes up in chunks of about
+// 64 MB through the two halves of a pinned buffer: host threads fill one half while the DMA and the
+// pack kernel of the previous chunk run on `st` (pageable copies reach about half the pinned
+// rate). Device data is packed in one chunk. `ready` is recorded after each chunk's pack.
+struct Upload {
+  knns_corpus_s* c = nullptr;
+  const void* data = nullptr;
+  bool dev = false;
+  hipStream_t st = nullptr;
+  long row_bytes = 0, chunk = 0, r0 = 0;
+  int ci = 0;
+  void* stage = nullptr;
+  char* hp[2] = {nullptr, nullptr};
+  hipEvent_t freed[2] = {nullptr, nullptr};   // DMA out of pinned half h done
+  bool pending[2] = {false, false};
+  hipEvent_t ready = nullptr;
+
+  knns_status start(knns_corpus_s* c_, const void* data_, bool dev_, hipStream_t st_) {
+    c = c_;
+    data = data_;
+    dev = dev_;
+    st = st_;
+    row_bytes = (long)c->d * (c->dtype == KNNS_F64 ? 8L : 4L);
+    chunk = dev ? c->rows_pad : std::min<long>(c->rows_pad, std::max<long>(1, (64L << 20) / row_bytes));
+    if (hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess) {
+      ready = nullptr;
+      return fail(KNNS_ERR_DEVICE, "hipEventCreate failed");
+    }
+    if (dev) return KNNS_OK;
+    if (hipMalloc(&stage, (size_t)chunk * row_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      stage = nullptr;
+      return fail(KNNS_ERR_NOMEM, "corpus staging buffer");
+    }
+    void* pp = nullptr;
+    if (knns_status ps = engine().upload_pinned(2 * (size_t)chunk * row_bytes, &pp)) return ps;
+    hp[0] = (char*)pp;
+    hp[1] = (char*)pp + (size_t)chunk * row_bytes;
+    for (auto& e : freed)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        e = nullptr;
+        return fail(KNNS_ERR_DEVICE, "hipEventCreate failed");
+      }
+    return KNNS_OK;
+  }
+
+  long rows_done() const { return r0; }
+
+  // enqueue the next chunk (no-op once every row is enqueued)
+  knns_status next() {
+    if (r0 >= c->rows_pad) return KNNS_OK;
+    const long rows_w = std::min(chunk, c->rows_pad - r0);
+    const long rows = std::max<long>(0, std::min(rows_w, c->m - r0));
+    const void* src = (const char*)data + (size_t)r0 * row_bytes;
+    const int h = ci & 1;
+    if (!dev && rows > 0) {
+      if (pending[h] && hipEventSynchronize(freed[h]) != hipSuccess)
+        return fail(KNNS_ERR_DEVICE, "corpus upload failed");
+      parallel_copy(hp[h], src, (size_t)rows * row_bytes);
+      // the DMA into `stage` follows the previous chunk's pack on the same stream
+      if (hipMemcpyAsync(stage, hp[h], (size_t)rows * row_bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipEventRecord(freed[h], st) != hipSuccess)
+        return fail(KNNS_ERR_DEVICE, "corpus upload failed");
+      pending[h] = true;
+      src = stage;
+    }
+    PackArgs pa{};
+    pa.src = src;
+    pa.src_f64 = c->dtype == KNNS_F64;
+    pa.rows = rows;
+    pa.rows_pad = rows_w;
+    pa.d = c->d;
+    pa.ld_src = c->d;
+    pa.x32 = c->x32 + (size_t)r0 * c->ld32;
+    pa.ld32 = c->ld32;
+    pa.x16 = c->x16 ? c->x16 + (size_t)r0 * c->ld16 : nullptr;
+    pa.ld16 = c->ld16;
+    pa.x64 = c->x64 ? c->x64 + (size_t)r0 * c->d : nullptr;
+    pa.dp = c->dp;
+    pa.is_query = 0;
+    pa.x8 = c->x8 ? c->x8 + (size_t)r0 * c->ld8 : nullptr;
+    pa.ld8 = c->ld8;
+    pa.dp8 = c->dp8;
+    if (launch_pack(pa, st) != hipSuccess || (c->x8 && launch_pack_fp8(pa, st) != hipSuccess))
+      return fail(KNNS_ERR_DEVICE, "corpus pack launch failed");
+    if (hipEventRecord(ready, st) != hipSuccess) return fail(KNNS_ERR_DEVICE, "hipEventRecord failed");
+    r0 += rows_w;
+    ++ci;
+    return KNNS_OK;
+  }
+
+  // wait for everything enqueued, release the staging (idempotent)
+  knns_status finish() {
+    knns_status s = KNNS_OK;
+    if (st && hipStreamSynchronize(st) != hipSuccess) s = fail(KNNS_ERR_DEVICE, "corpus pack failed");
+    st = nullptr;
+    if (stage) (void)hipFree(stage);
+    stage = nullptr;
+    for (auto& e : freed)
+      if (e) (void)hipEventDestroy(e);
+    freed[0] = freed[1] = nullptr;
+    if (ready) (void)hipEventDestroy(ready);
+    ready = nullptr;
+    return s;
+  }
+
+  ~Upload() { (void)finish(); }
+};
+
+// Enqueue the upload until `rows` rows (capped at rows_pad) are packed; `st` waits for them.
+hipError_t feed_rows(Upload& u, long rows, hipStream_t st) {
+  rows = std::min(rows, u.c->rows_pad);
+  while (u.rows_done() < rows)
+    if (u.next() != KNNS_OK) return hipErrorUnknown;
+  return hipStreamWaitEvent(st, u.ready, 0);
+}
+
 hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStream_t st,
                           const LaunchHook* hook) {
   const knns_corpus_s* c = L.c;
@@ -416,6 +558,8 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     sa.flags = (int*)B.flags;
     sa.dense = R.dense;
     sa.n_dev = L.n_dev;
+    // an overlapped upload: the scan reads up to one stage (<= 128 rows) past its round's end
+    if (L.feed) KNNS_TRY(feed_rows(*L.feed, R.end + 128, st));
     if (hook) (*hook)(0);
     KNNS_TRY(launch_scan(P.prec_kernel, P.qb, P.dc, sa, P.groups, st));
     if (hook) (*hook)(1);
@@ -461,6 +605,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   fa.out_idx = (int32_t*)B.oidx;
   fa.n = (int)n;
   fa.n_dev = L.n_dev;
+  if (L.feed) KNNS_TRY(feed_rows(*L.feed, c->rows_pad, st));
   if (hook) (*hook)(3);
   KNNS_TRY(launch_final(fa, st));
   if (hook) (*hook)(4);
@@ -482,7 +627,7 @@ bool is_device_ptr(const void* p) {
 // Searches corpus rows [row_begin, row_end) (row_end < 0: to the last row); indices are corpus rows.
 knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, const int* kq_host,
                       int kuni, const knns_params* p, void* dist, int32_t* idx,
-                      long row_begin = 0, long row_end = -1) {
+                      long row_begin = 0, long row_end = -1, Upload* feed = nullptr) {
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lock(E.mu);
   g_err.clear();
@@ -630,6 +775,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   L.self_offset = p->self_offset;
   L.out64 = out64;
   L.bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
+  L.feed = feed;
   // per-launch events (KNNS_KERNEL_TIMES): pairs (scan begin, scan end), (scan end, merge end),
   // (final begin, final end)
   std::vector<std::pair<size_t, size_t>> scan_ev, sel_ev;
@@ -684,29 +830,6 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   return KNNS_OK;
 }
 
-// memcpy split over up to 8 host threads (one thread moves about 30 GB/s on these hosts, the DMA
-// from pinned memory about 57 GB/s)
-void parallel_copy(void* dst, const void* src, size_t bytes) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t parts = std::min<size_t>(std::min(hw, 8u), bytes >> 22);   // >= 4 MB per thread
-  if (parts <= 1) {
-    std::memcpy(dst, src, bytes);
-    return;
-  }
-  const size_t step = ((bytes + parts - 1) / parts + 4095) & ~(size_t)4095;
-  std::vector<std::thread> th;
-  th.reserve(parts);
-  try {
-    for (size_t o = step; o < bytes; o += step)
-      th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, std::min(step, bytes - o)); });
-  } catch (...) {   // no thread: the caller's thread copies what is left
-    size_t o = step * (th.size() + 1);
-    if (o < bytes) std::memcpy((char*)dst + o, (const char*)src + o, bytes - o);
-  }
-  std::memcpy(dst, src, std::min(step, bytes));
-  for (auto& t : th) t.join();
-}
-
 }  // namespace
 }  // namespace knns
 
@@ -722,11 +845,15 @@ void knns_params_init(knns_params* p, int precision, int k) {
   p->k = k;
 }
 
-knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, long m, int d,
-                               unsigned flags) {
+}  // extern "C"
+
+namespace knns {
+namespace {
+
+// Validation and device buffers of a corpus; its rows are written by an Upload.
+knns_status corpus_alloc(knns_corpus* out, const void* data, int dtype, long m, int d,
+                         unsigned flags) {
   Engine& E = engine();
-  std::lock_guard<std::recursive_mutex> lock(E.mu);
-  g_err.clear();
   if (!out) return fail(KNNS_ERR_INVALID, "null output handle");
   *out = nullptr;
   if (!data) return fail(KNNS_ERR_INVALID, "null corpus data");
@@ -769,81 +896,31 @@ knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, lo
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP8 rows"));
   if (dtype == KNNS_F64 && hipMalloc((void**)&c->x64, (size_t)m * d * 8) != hipSuccess)
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP64 master"));
-  const size_t esz = dtype == KNNS_F64 ? 8 : 4;
-  const bool dev = (flags & KNNS_CORPUS_DEVICE_PTR) != 0;
-  // Host rows go up in chunks of about 64 MB through the two halves of a pinned buffer: host
-  // threads fill one half while the DMA and the pack kernel of the previous chunk run (pageable
-  // copies reach about half the pinned rate).
-  const long row_bytes = (long)d * (long)esz;
-  const long chunk = dev ? c->rows_pad
-                         : std::min<long>(c->rows_pad, std::max<long>(1, (64L << 20) / row_bytes));
-  void* stage = nullptr;
-  if (!dev && hipMalloc(&stage, (size_t)chunk * row_bytes) != hipSuccess)
-    return cleanup(fail(KNNS_ERR_NOMEM, "corpus staging buffer"));
-  char* hp[2] = {nullptr, nullptr};
-  hipEvent_t freed[2] = {nullptr, nullptr};   // DMA out of half h done
-  bool pending[2] = {false, false};
-  auto end_upload = [&](knns_status st) {
-    if (stage) (void)hipFree(stage);
-    for (hipEvent_t e : freed)
-      if (e) (void)hipEventDestroy(e);
-    return st == KNNS_OK ? st : cleanup(st);
-  };
-  if (!dev) {
-    void* pp = nullptr;
-    if (knns_status ps = E.upload_pinned(2 * (size_t)chunk * row_bytes, &pp))
-      return end_upload(ps);
-    hp[0] = (char*)pp;
-    hp[1] = (char*)pp + (size_t)chunk * row_bytes;
-    for (auto& e : freed)
-      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-        return end_upload(fail(KNNS_ERR_DEVICE, "hipEventCreate failed"));
+  *out = c;
+  return KNNS_OK;
+}
+
+}  // namespace
+}  // namespace knns
+
+extern "C" {
+
+knns_status knns_corpus_create(knns_corpus* out, const void* data, int dtype, long m, int d,
+                               unsigned flags) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lock(E.mu);
+  g_err.clear();
+  knns_corpus c = nullptr;
+  if (knns_status st = corpus_alloc(&c, data, dtype, m, d, flags)) return st;
+  Upload u;
+  knns_status st = u.start(c, data, (flags & KNNS_CORPUS_DEVICE_PTR) != 0, E.stream);
+  while (st == KNNS_OK && u.rows_done() < c->rows_pad) st = u.next();
+  const knns_status fs = u.finish();
+  if (st == KNNS_OK) st = fs;
+  if (st != KNNS_OK) {
+    knns_corpus_destroy(c);
+    return st;
   }
-  hipStream_t st = E.stream;
-  int ci = 0;
-  for (long r0 = 0; r0 < c->rows_pad; r0 += chunk, ++ci) {
-    const long rows_w = std::min(chunk, c->rows_pad - r0);
-    const long rows = std::max<long>(0, std::min(rows_w, m - r0));
-    const void* src = (const char*)data + (size_t)r0 * row_bytes;
-    const int h = ci & 1;
-    if (!dev && rows > 0) {
-      if (pending[h] && hipEventSynchronize(freed[h]) != hipSuccess)
-        return end_upload(fail(KNNS_ERR_DEVICE, "corpus upload failed"));
-      parallel_copy(hp[h], src, (size_t)rows * row_bytes);
-      if (hipMemcpyAsync(stage, hp[h], (size_t)rows * row_bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
-          hipEventRecord(freed[h], st) != hipSuccess) {
-        (void)hipStreamSynchronize(st);
-        return end_upload(fail(KNNS_ERR_DEVICE, "corpus upload failed"));
-      }
-      pending[h] = true;
-      src = stage;
-    }
-    PackArgs pa{};
-    pa.src = src;
-    pa.src_f64 = dtype == KNNS_F64;
-    pa.rows = rows;
-    pa.rows_pad = rows_w;
-    pa.d = d;
-    pa.ld_src = d;
-    pa.x32 = c->x32 + (size_t)r0 * c->ld32;
-    pa.ld32 = c->ld32;
-    pa.x16 = c->x16 ? c->x16 + (size_t)r0 * c->ld16 : nullptr;
-    pa.ld16 = c->ld16;
-    pa.x64 = c->x64 ? c->x64 + (size_t)r0 * d : nullptr;
-    pa.dp = c->dp;
-    pa.is_query = 0;
-    pa.x8 = c->x8 ? c->x8 + (size_t)r0 * c->ld8 : nullptr;
-    pa.ld8 = c->ld8;
-    pa.dp8 = c->dp8;
-    // the next chunk's DMA into `stage` follows this pack on the same stream
-    if (launch_pack(pa, st) != hipSuccess || (c->x8 && launch_pack_fp8(pa, st) != hipSuccess)) {
-      (void)hipStreamSynchronize(st);
-      return end_upload(fail(KNNS_ERR_DEVICE, "corpus pack launch failed"));
-    }
-  }
-  if (hipStreamSynchronize(st) != hipSuccess)
-    return end_upload(fail(KNNS_ERR_DEVICE, "corpus pack failed"));
-  end_upload(KNNS_OK);
   *out = c;
   return KNNS_OK;
 }
@@ -883,6 +960,26 @@ knns_status knns_search_rows(knns_corpus c, long row_begin, long row_end, const 
     return fail(KNNS_ERR_INVALID, "row range [%ld, %ld) outside the corpus' %ld rows", row_begin, row_end,
                 c->m);
   return do_search(c, queries, dtype, n, nullptr, p->k, p, dist, idx, row_begin, row_end);
+}
+
+knns_status knns_search_once(const void* data, int dtype, long m, int d, const void* queries,
+                             long n, const knns_params* p, void* dist, int32_t* idx) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lock(E.mu);
+  g_err.clear();
+  if (!p) return fail(KNNS_ERR_INVALID, "null params");
+  if (p->flags & KNNS_DEVICE_PTRS) return fail(KNNS_ERR_INVALID, "knns_search_once takes host buffers");
+  knns_corpus c = nullptr;
+  if (knns_status st = corpus_alloc(&c, data, dtype, m, d, 0)) return st;
+  // the corpus goes up on the upload stream; each scan round waits only for the rows it reads
+  Upload u;
+  knns_status st = u.start(c, data, false, E.ustream);
+  if (st == KNNS_OK) st = u.next();
+  if (st == KNNS_OK) st = do_search(c, queries, dtype, n, nullptr, p->k, p, dist, idx, 0, -1, &u);
+  const knns_status fs = u.finish();
+  if (st == KNNS_OK) st = fs;
+  knns_corpus_destroy(c);
+  return st;
 }
 
 knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns_stats* out) {
