@@ -104,7 +104,8 @@ knns_status knns_search_rows(knns_corpus c, long row_begin, long row_end, const 
  * (the whole-problem call of gpuknnsBitonic / gpuknnsHeap, src/cuknnBitonic.cu:422-618): host rows
  * (m x d of `dtype`) upload on a second stream while the first threshold rounds, which scan a
  * growing prefix of the corpus, already run. Host queries and outputs (KNNS_DEVICE_PTRS is
- * rejected); results as knns_search. */
+ * rejected); precision FP32 or FP64 (no low-precision corpus copy is built); results as
+ * knns_search. */
 knns_status knns_search_once(const void* data, int dtype, long m, int d, const void* queries,
                              long n, const knns_params* p, void* dist, int32_t* idx);
 
