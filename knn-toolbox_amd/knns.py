@@ -106,6 +106,8 @@ def native():
         L.knns_search_multi.restype = I
         L.knns_search_rows.argtypes = [V, L_, L_, V, I, L_, ctypes.POINTER(Params), V, V]
         L.knns_search_rows.restype = I
+        L.knns_search_once.argtypes = [V, I, L_, I, V, L_, ctypes.POINTER(Params), V, V]
+        L.knns_search_once.restype = I
         L.knns_plan_describe.argtypes = [L_, I, L_, I, I, ctypes.POINTER(Stats)]
         L.knns_plan_describe.restype = I
         L.knns_last_error.argtypes = []

@@ -110,6 +110,13 @@ def test_native_argument_errors_without_gpu(knns_mod):
     assert L.knns_search_rows(None, 0, 4, x.ctypes.data, 0, 4, ctypes.byref(p), x.ctypes.data,
                               x.ctypes.data) == knns_mod.KNNS_ERR_INVALID
     assert b"null corpus" in L.knns_last_error()
+    # the one-shot call takes host buffers only; both checks run before any device work
+    p.flags = knns_mod.KNNS_DEVICE_PTRS
+    assert L.knns_search_once(x.ctypes.data, 0, 4, 4, x.ctypes.data, 4, ctypes.byref(p), x.ctypes.data,
+                              x.ctypes.data) == knns_mod.KNNS_ERR_INVALID
+    assert b"host buffers" in L.knns_last_error()
+    assert L.knns_search_once(x.ctypes.data, 0, 4, 4, x.ctypes.data, 4, None, x.ctypes.data,
+                              x.ctypes.data) == knns_mod.KNNS_ERR_INVALID
 
 
 def test_dropin_rejects_bad_sizes_without_gpu(knns_mod):
