@@ -67,7 +67,9 @@ float gpuknnLSH(knntype* query, knntype* data, knntype* values, knntype* indices
                 int objects, int numQueries, int attributes, int k, int numStreams,
                 int* bucketSize, int* query_offsets, int c, int* query_sizes);
 
-/* cuknns.h:85. Table-driven: no trial runs on random data (src/knnsplan.cu:43-57 ran 80). */
+/* cuknns.h:85. A fixed rule, no trial runs on random data (src/knnsplan.cu:43-57 ran 80):
+ * gpuknnsHeap with one stream (the two entry points return identical results and the heap form
+ * measured faster at every size tried). */
 void knnsplan(knnplan* plan, long int N, long int Q, long int D, long int k);
 
 /* cuknns.h:87. Prints "Time Elapsed: %f\n" like src/knnsexecute.cu:16 (KNNS_QUIET=1 silences). */
@@ -75,9 +77,13 @@ void knnsexecute(knnplan plan, knntype* data, knntype* queries, knntype* KNNdist
                  knntype* KNNidx);
 
 /* Additions with no counterpart in the reference: the reason of the last failure (empty string
- * after a success) and the stage timers of the last call. */
+ * after a success) and the timers of the last call. knn_time (the whole call) is always set;
+ * dst_time and srch_time (distance and search stages) only while stage timing is on, because it
+ * records a device event around every kernel launch of the call. Stage timing is off unless
+ * knnsSetStageTimes(1) was called or the environment holds KNNS_STAGE_TIMES=1. */
 const char* knnsGetLastError(void);
 knntimes knnsGetLastTimes(void);
+void knnsSetStageTimes(int enable);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,8 @@ typedef enum {
   KNNS_ERR_NOMEM = 2,        /* device or host allocation failed, or over the memory budget */
   KNNS_ERR_DEVICE = 3,       /* HIP runtime error or no device */
   KNNS_ERR_UNSUPPORTED = 4,  /* valid request outside what this library implements */
-  KNNS_ERR_OVERFLOW = 5      /* candidate pool overflow (massive exact ties); results invalid */
+  KNNS_ERR_OVERFLOW = 5,     /* candidate pool overflow (massive exact ties); results invalid */
+  KNNS_ERR_TIMEOUT = 6       /* knns_server_wait: not done within the timeout (the request stays live) */
 } knns_status;
 
 typedef enum { KNNS_F32 = 0, KNNS_F64 = 1 } knns_dtype;
@@ -155,7 +156,9 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
 knns_status knns_server_submit(knns_server s, const void* queries, int dtype, long n, int k,
                                void* dist, int32_t* idx, unsigned long long* ticket);
 /* Blocks until the request is done (timeout_s < 0: no limit) and returns its status; a ticket
- * can be waited for once. KNNS_ERR_INVALID for an unknown ticket, KNNS_ERR_DEVICE on timeout. */
+ * can be waited for once it is done. KNNS_ERR_INVALID for an unknown ticket, KNNS_ERR_TIMEOUT
+ * when the timeout runs out first: the request stays live, its output buffers must stay valid,
+ * and the ticket can be waited for again. */
 knns_status knns_server_wait(knns_server s, unsigned long long ticket, double timeout_s);
 knns_status knns_server_stats_get(knns_server s, knns_server_stats* out, int reset);
 /* Finishes the requests already submitted, then frees the server. */

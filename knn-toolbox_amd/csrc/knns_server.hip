@@ -379,7 +379,7 @@ knns_status knns_server_wait(knns_server s, unsigned long long ticket, double ti
   if (timeout_s < 0) {
     s->cv_done.wait(lk, ready);
   } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
-    return fail(KNNS_ERR_DEVICE, "request %llu not done after %.3f s", ticket, timeout_s);
+    return fail(KNNS_ERR_TIMEOUT, "request %llu not done after %.3f s", ticket, timeout_s);
   }
   s->live.erase(ticket);
   if (r->status != KNNS_OK) return fail(r->status, "%s", r->err.c_str());

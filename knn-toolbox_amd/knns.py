@@ -22,14 +22,15 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 LIB_DIR = PKG / "lib"
 
-KNNS_OK, KNNS_ERR_INVALID, KNNS_ERR_NOMEM, KNNS_ERR_DEVICE, KNNS_ERR_UNSUPPORTED, KNNS_ERR_OVERFLOW = range(6)
+(KNNS_OK, KNNS_ERR_INVALID, KNNS_ERR_NOMEM, KNNS_ERR_DEVICE, KNNS_ERR_UNSUPPORTED, KNNS_ERR_OVERFLOW,
+ KNNS_ERR_TIMEOUT) = range(7)
 KNNS_F32, KNNS_F64 = 0, 1
 PREC = {"fp32": 0, "bf16": 1, "fp8": 2, "fp64": 3}
 SELECT = {"auto": 0, "threshold": 1, "heap": 1, "bitonic": 2}
 KNNS_CORPUS_DEVICE_PTR, KNNS_CORPUS_BF16, KNNS_CORPUS_FP8 = 0x1, 0x2, 0x4
 KNNS_DEVICE_PTRS, KNNS_EXCLUDE_SELF, KNNS_DIST_F64, KNNS_KERNEL_TIMES = 0x1, 0x2, 0x4, 0x8
 
-STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "OVERFLOW"}
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "OVERFLOW", 6: "TIMEOUT"}
 
 
 class KnnsError(RuntimeError):
@@ -155,6 +156,8 @@ def dropin(double: bool = False):
         L.knnsGetLastError.restype = ctypes.c_char_p
         L.knnsGetLastTimes.argtypes = []
         L.knnsGetLastTimes.restype = Times
+        L.knnsSetStageTimes.argtypes = [I]
+        L.knnsSetStageTimes.restype = None
         _dropin[double] = L
     return _dropin[double]
 
@@ -327,6 +330,17 @@ def _dropin_call(fn_name: str, query, data, k: int, double: bool, num_streams: i
     return vals, inds, t
 
 
+def dropin_stage_times(enable: bool, double: bool = False) -> None:
+    """knnsSetStageTimes: per-stage device times in knnsGetLastTimes (off by default)."""
+    dropin(double).knnsSetStageTimes(1 if enable else 0)
+
+
+def dropin_last_times(double: bool = False) -> dict:
+    """knnsGetLastTimes of the drop-in library: {dst_time, srch_time, knn_time} in ms."""
+    t = dropin(double).knnsGetLastTimes()
+    return {"dst_time": t.dst_time, "srch_time": t.srch_time, "knn_time": t.knn_time}
+
+
 def gpuknns_bitonic(query, data, k: int, double: bool = False, num_streams: int = 1):
     """gpuknnsBitonic(query, data, values, indices, objects, numQueries, attributes, k, numStreams)."""
     return _dropin_call("gpuknnsBitonic", query, data, k, double, num_streams)
@@ -415,8 +429,15 @@ class Server:
         return t.value
 
     def result(self, ticket: int, timeout: float = -1.0):
-        D, I = self._out.pop(ticket)
+        """Blocks until the request is done (timeout < 0: no limit). On a timeout the request
+        stays live and its output arrays stay referenced here (the worker still writes them), so
+        result() can be called again for the same ticket."""
+        if ticket not in self._out:
+            raise KnnsError(KNNS_ERR_INVALID, f"unknown ticket {ticket}")
         st = native().knns_server_wait(self._h, ticket, float(timeout))
+        if st == KNNS_ERR_TIMEOUT:
+            raise TimeoutError(f"request {ticket} not done after {timeout} s")
+        D, I = self._out.pop(ticket)
         if st:
             _err(st)
         return D, I

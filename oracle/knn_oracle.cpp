@@ -35,6 +35,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -46,28 +47,87 @@ T seq_dot(const T* a, const T* b, int d) {
   return acc;
 }
 
+// Order-preserving unsigned image of a key (equal keys -> equal images: -0 is folded onto +0).
+inline uint32_t key_bits(float f) {
+  if (f == 0.0f) f = 0.0f;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+inline uint64_t key_bits(double f) {
+  if (f == 0.0) f = 0.0;
+  uint64_t u;
+  std::memcpy(&u, &f, 8);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// Stable ascending sort of perm[0..m) by key[perm[i]]: LSD radix sort over the order-preserving
+// key images, 8 bits per pass. Every pass is stable, so equal keys keep their input order -- the
+// result is the permutation std::stable_sort(perm, key[a] < key[b]) gives (MATLAB's sort).
+template <typename T>
+void stable_key_sort(const T* key, int32_t* perm, int32_t* tmp, long m) {
+  using U = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+  std::vector<U> kb(m), kt(m);
+  for (long j = 0; j < m; ++j) kb[j] = key_bits(key[perm[j]]);
+  for (unsigned sh = 0; sh < 8 * sizeof(U); sh += 8) {
+    long cnt[257] = {0};
+    for (long j = 0; j < m; ++j) ++cnt[((kb[j] >> sh) & 0xff) + 1];
+    if (cnt[1 + ((kb[0] >> sh) & 0xff)] == m) continue;   // one bucket: the pass is the identity
+    for (int b = 0; b < 256; ++b) cnt[b + 1] += cnt[b];
+    for (long j = 0; j < m; ++j) {
+      const long o = cnt[(kb[j] >> sh) & 0xff]++;
+      tmp[o] = perm[j];
+      kt[o] = kb[j];
+    }
+    std::memcpy(perm, tmp, m * sizeof(int32_t));
+    kb.swap(kt);
+  }
+}
+
 template <typename T>
 void knnm(const T* X, const T* Y, long n, long m, int d, int k, T* Dout, int32_t* Iout) {
   // dot(Y,Y,1)' : corpus norms (knn.m:5)
   std::vector<T> yy(m);
   for (long j = 0; j < m; ++j) yy[j] = seq_dot(Y + j * d, Y + j * d, d);
+  // The GEMM is blocked (QB queries x JT corpus rows per tile, the tile transposed once per query
+  // block) but every dot product keeps its own sequential order over the d terms, so each key is
+  // bit-identical to seq_dot's: the blocking only vectorises across independent dot products.
+  constexpr int QB = 8, JT = 64;
 #pragma omp parallel
   {
-    std::vector<T> key(m);
-    std::vector<int32_t> perm(m);
+    std::vector<T> key((size_t)QB * m);
+    std::vector<int32_t> perm(m), tmp(m);
+    std::vector<T> yt((size_t)d * JT);
 #pragma omp for schedule(dynamic, 1)
-    for (long i = 0; i < n; ++i) {
-      const T* x = X + i * d;
-      // (-2)*(Y'*X) + dot(Y,Y,1)'  (one column of the bsxfun result)
-      for (long j = 0; j < m; ++j) key[j] = (T(-2) * seq_dot(Y + j * d, x, d)) + yy[j];
-      std::iota(perm.begin(), perm.end(), 0);
-      // MATLAB sort: ascending and stable
-      std::stable_sort(perm.begin(), perm.end(),
-                       [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-      const T xx = seq_dot(x, x, d);  // dot(X,X,1) (knn.m:9)
-      for (int t = 0; t < k; ++t) {
-        Dout[i * k + t] = key[perm[t]] + xx;
-        Iout[i * k + t] = perm[t];
+    for (long i0 = 0; i0 < n; i0 += QB) {
+      const int nq = (int)std::min<long>(QB, n - i0);
+      for (long j0 = 0; j0 < m; j0 += JT) {
+        const int nj = (int)std::min<long>(JT, m - j0);
+        for (int jj = 0; jj < JT; ++jj)
+          for (int t = 0; t < d; ++t) yt[(size_t)t * JT + jj] = jj < nj ? Y[(j0 + jj) * d + t] : T(0);
+        for (int qi = 0; qi < nq; ++qi) {
+          const T* x = X + (i0 + qi) * d;
+          T acc[JT];
+          for (int jj = 0; jj < JT; ++jj) acc[jj] = T(0);
+          for (int t = 0; t < d; ++t) {
+            const T xt = x[t];
+            const T* yr = &yt[(size_t)t * JT];
+            for (int jj = 0; jj < JT; ++jj) acc[jj] = acc[jj] + yr[jj] * xt;
+          }
+          // (-2)*(Y'*X) + dot(Y,Y,1)'  (one column of the bsxfun result)
+          for (int jj = 0; jj < nj; ++jj) key[(size_t)qi * m + j0 + jj] = (T(-2) * acc[jj]) + yy[j0 + jj];
+        }
+      }
+      for (int qi = 0; qi < nq; ++qi) {
+        const long i = i0 + qi;
+        const T* kq = &key[(size_t)qi * m];
+        std::iota(perm.begin(), perm.end(), 0);
+        stable_key_sort(kq, perm.data(), tmp.data(), m);   // MATLAB sort: ascending and stable
+        const T xx = seq_dot(X + i * d, X + i * d, d);     // dot(X,X,1) (knn.m:9)
+        for (int t = 0; t < k; ++t) {
+          Dout[i * k + t] = kq[perm[t]] + xx;
+          Iout[i * k + t] = perm[t];
+        }
       }
     }
   }
@@ -135,13 +195,17 @@ void oracle_exact_topk(const void* Q, const void* R, int dtype, long n, long m, 
     exact_topk<double>((const double*)Q, (const double*)R, n, m, d, k, self, D, I);
 }
 
-// Canonical distances for explicit (query, candidate) pairs: D[i*c + t] = dist(Q[i], R[C[i*c+t]]).
-void oracle_pair_dist(const void* Q, const void* R, int dtype, long n, int d, int c,
+// Canonical distances for explicit (query, candidate) pairs: D[i*c + t] = dist(Q[i], R[C[i*c+t]]),
+// Q and R both of `dtype`, R holding m rows. A negative index (the -1 of an empty slot past a
+// bucket smaller than k) gives +inf; an index >= m is never read and gives NaN.
+void oracle_pair_dist(const void* Q, const void* R, int dtype, long n, long m, int d, int c,
                       const int32_t* C, double* D) {
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < n; ++i)
     for (int t = 0; t < c; ++t) {
       const long j = C[i * c + t];
+      if (j < 0) { D[i * c + t] = HUGE_VAL; continue; }
+      if (j >= m) { D[i * c + t] = std::nan(""); continue; }
       D[i * c + t] = dtype == 0 ? canon_dist((const float*)Q + i * d, (const float*)R + j * d, d)
                                 : canon_dist((const double*)Q + i * d, (const double*)R + j * d, d);
     }

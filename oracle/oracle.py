@@ -62,7 +62,7 @@ def load():
         lib.oracle_knnm_f32.argtypes = [P, P, L, L, I, I, P, P]
         lib.oracle_knnm_f64.argtypes = [P, P, L, L, I, I, P, P]
         lib.oracle_exact_topk.argtypes = [P, P, I, L, L, I, I, P, P, P]
-        lib.oracle_pair_dist.argtypes = [P, P, I, L, I, I, P, P]
+        lib.oracle_pair_dist.argtypes = [P, P, I, L, L, I, I, P, P]
         lib.oracle_keys_f32.argtypes = [P, P, L, L, I, P]
         _lib = lib
     return _lib
@@ -132,14 +132,23 @@ def bucketed_topk(Q: np.ndarray, R: np.ndarray, k: int, bucket_sizes, query_offs
 
 
 def pair_dist(Q: np.ndarray, R: np.ndarray, C: np.ndarray) -> np.ndarray:
-    """Canonical FP64 distance for the explicit pairs (i, C[i, t])."""
+    """Canonical FP64 distance for the explicit pairs (i, C[i, t]): the expression of
+    ``exact_topk``, evaluated for the candidates a search returned. C[i, t] = -1 (an empty slot)
+    gives +inf. Q and R must share a dtype (float32 or float64); indices >= len(R) are rejected."""
     lib = load()
     Q = np.ascontiguousarray(Q)
     R = np.ascontiguousarray(R)
+    if Q.dtype != R.dtype or Q.dtype not in (np.float32, np.float64):
+        raise TypeError(f"pair_dist: Q ({Q.dtype}) and R ({R.dtype}) must both be float32 or float64")
     C = np.ascontiguousarray(C, dtype=np.int32)
     n, d = Q.shape
+    m = R.shape[0]
+    if C.ndim != 2 or C.shape[0] != n or R.shape[1] != d:
+        raise ValueError(f"pair_dist: shapes Q {Q.shape}, R {R.shape}, C {C.shape} do not match")
+    if C.size and int(C.max()) >= m:
+        raise ValueError(f"pair_dist: candidate index {int(C.max())} outside the {m} corpus rows")
     D = np.empty(C.shape, dtype=np.float64)
-    lib.oracle_pair_dist(_p(Q), _p(R), 0 if Q.dtype == np.float32 else 1, n, d, C.shape[1], _p(C), _p(D))
+    lib.oracle_pair_dist(_p(Q), _p(R), 0 if Q.dtype == np.float32 else 1, n, m, d, C.shape[1], _p(C), _p(D))
     return D
 
 
@@ -153,14 +162,21 @@ def keys_f32(Q: np.ndarray, R: np.ndarray) -> np.ndarray:
     return K
 
 
-def canonical_dist_np(q: np.ndarray, r: np.ndarray) -> float:
-    """Pure-Python canonical distance for one pair (small cases; pins the C++ one)."""
-    import math
+def _fma(a: float, b: float, c: float) -> float:
+    """Correctly rounded a*b + c (math.fma exists only from Python 3.13): the exact rational
+    value, rounded once to the nearest double (Fraction -> float rounds half to even)."""
+    from fractions import Fraction
 
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def canonical_dist_np(q: np.ndarray, r: np.ndarray) -> float:
+    """Pure-Python canonical distance for one pair (small cases; pins the C++ one):
+    acc = fma(t_i, t_i, acc) over the dims in order, t_i = double(q_i) - double(r_i)."""
     acc = 0.0
     for a, b in zip(q.tolist(), r.tolist()):
         t = float(a) - float(b)
-        acc = math.fma(t, t, acc) if hasattr(math, "fma") else acc + t * t
+        acc = _fma(t, t, acc)
     return acc
 
 
@@ -441,14 +457,22 @@ def fp8_keys(Q: np.ndarray, R: np.ndarray) -> np.ndarray:
 # Metrics
 # ----------------------------------------------------------------------------------------------
 def recall_at_k(I_got: np.ndarray, I_true: np.ndarray) -> float:
+    """recall@k = sum_i |set(I_got[i, :k]) & set(I_true[i])| / (n k), k = I_true.shape[1]. A
+    neighbour returned twice counts once; -1 (an empty slot) never matches a true neighbour."""
     n, k = I_true.shape
     hit = 0
     for i in range(n):
-        hit += len(np.intersect1d(I_got[i, :k], I_true[i]))
+        got = np.asarray(I_got[i, :k])
+        hit += len(np.intersect1d(got[got >= 0], I_true[i]))
     return hit / float(n * k)
 
 
 def max_rel_err(D_got: np.ndarray, D_true: np.ndarray) -> float:
+    """max over entries of |D_got - D_true| / |D_true| (the denominator floored at the smallest
+    normal double, so an exact zero distance compares absolutely); 0 for empty arrays."""
     D_got = np.asarray(D_got, dtype=np.float64)
+    D_true = np.asarray(D_true, dtype=np.float64)
+    if D_got.shape != D_true.shape:
+        raise ValueError(f"max_rel_err: shapes {D_got.shape} and {D_true.shape} differ")
     den = np.maximum(np.abs(D_true), np.finfo(np.float64).tiny)
     return float(np.max(np.abs(D_got - D_true) / den)) if D_true.size else 0.0

@@ -374,3 +374,23 @@ def test_large_corpus_row_offsets(gpu, knns_mod, oracle_mod):
             dd, ii = c.search(Q, k, precision=prec)
             assert np.array_equal(ii.astype(np.int64), I.astype(np.int64)), prec
             assert np.array_equal(dd, D.astype(np.float32)), prec
+
+
+@pytest.mark.gpu
+def test_dropin_stage_times_are_opt_in(gpu, knns_mod, oracle_mod):
+    """Per-kernel device events cost time, so the drop-in records stage times only on request
+    (knnsSetStageTimes); the whole-call time is always there, and results do not change."""
+    R = oracle_mod.uniform(20000, 64, 71)
+    Q = oracle_mod.uniform(200, 64, 72)
+    try:
+        knns_mod.dropin_stage_times(False)
+        v0, i0, _ = knns_mod.gpuknns_heap(Q, R, 10)
+        t0 = knns_mod.dropin_last_times()
+        assert t0["knn_time"] > 0 and t0["dst_time"] == 0 and t0["srch_time"] == 0
+        knns_mod.dropin_stage_times(True)
+        v1, i1, _ = knns_mod.gpuknns_heap(Q, R, 10)
+        t1 = knns_mod.dropin_last_times()
+        assert t1["dst_time"] > 0 and t1["srch_time"] > 0 and t1["knn_time"] > 0
+    finally:
+        knns_mod.dropin_stage_times(False)
+    assert np.array_equal(i0, i1) and np.array_equal(v0, v1)

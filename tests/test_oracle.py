@@ -270,3 +270,107 @@ def test_bucketed_topk_pinned(oracle_mod):
     assert np.array_equal(I, Ib) and np.array_equal(D, Db)
     assert np.all(I[12:20, 5:] == -1) and np.all(I[12:20, :5] >= 150) and np.all(I[12:20, :5] < 155)
     assert np.all((I[20:30] >= 155) & (I[20:30] < 225))
+
+
+# ---- pins of the helpers the GPU tests assert through: pair_dist, canonical_dist_np, recall_at_k,
+# max_rel_err (each checked against values computed independently of the helper itself) ----
+def test_pair_dist_matches_integer_brute_force_on_shuffled_candidates(oracle_mod):
+    rng = np.random.default_rng(41)
+    n, m, d, c = 7, 90, 11, 5          # n, m, d, c all different: a transposed index fails
+    R = _int_data(rng, m, d)
+    Q = _int_data(rng, n, d)
+    C = np.stack([rng.permutation(m)[:c] for _ in range(n)]).astype(np.int32)
+    D = oracle_mod.pair_dist(Q, R, C)
+    Qi, Ri = Q.astype(np.int64).tolist(), R.astype(np.int64).tolist()
+    want = [[sum((a - b) ** 2 for a, b in zip(Qi[i], Ri[C[i, t]])) for t in range(c)] for i in range(n)]
+    assert np.array_equal(D, np.array(want, dtype=np.float64))
+    # the same pairs in double precision, and the pairs exact_topk ranks first
+    assert np.array_equal(oracle_mod.pair_dist(Q.astype(np.float64), R.astype(np.float64), C), D)
+    Dt, It = oracle_mod.exact_topk(Q, R, c)
+    assert np.array_equal(oracle_mod.pair_dist(Q, R, It), Dt)
+
+
+def test_pair_dist_float_pairs_equal_the_python_canonical_form(oracle_mod):
+    rng = np.random.default_rng(42)
+    R = oracle_mod.uniform(64, 19, 43)
+    Q = oracle_mod.uniform(3, 19, 44)
+    C = np.array([[5, 63, 0, 5], [17, 2, 40, 41], [1, 1, 60, 33]], dtype=np.int32)
+    D = oracle_mod.pair_dist(Q, R, C)
+    for i in range(3):
+        for t in range(4):
+            assert D[i, t] == oracle_mod.canonical_dist_np(Q[i], R[C[i, t]])
+
+
+def test_pair_dist_empty_slots_and_bad_arguments(oracle_mod):
+    R = np.arange(12, dtype=np.float32).reshape(4, 3)
+    Q = np.zeros((2, 3), dtype=np.float32)
+    D = oracle_mod.pair_dist(Q, R, np.array([[-1, 0], [3, -1]], dtype=np.int32))
+    assert np.isinf(D[0, 0]) and np.isinf(D[1, 1])
+    assert D[0, 1] == 0 + 1 + 4 and D[1, 0] == 81 + 100 + 121
+    with pytest.raises(ValueError):
+        oracle_mod.pair_dist(Q, R, np.array([[4, 0], [0, 0]], dtype=np.int32))
+    with pytest.raises(TypeError):
+        oracle_mod.pair_dist(Q, R.astype(np.float64), np.zeros((2, 1), dtype=np.int32))
+
+
+def test_canonical_dist_np_is_the_sequential_fma_chain():
+    import math
+    from fractions import Fraction
+
+    import oracle
+
+    # integers: the exact sum
+    q = np.array([3, -2, 7, 0], dtype=np.float32)
+    r = np.array([-1, 5, 7, 2], dtype=np.float32)
+    assert oracle.canonical_dist_np(q, r) == 16 + 49 + 0 + 4
+    # floats: fma keeps every product exact, so the result is the exact rational sum rounded at
+    # each step; within d ulps of the exact value, and equal to an explicit left-to-right fold
+    rng = np.random.default_rng(45)
+    q = rng.random(33).astype(np.float32)
+    r = rng.random(33).astype(np.float32)
+    exact = sum((Fraction(float(a)) - Fraction(float(b))) ** 2 for a, b in zip(q, r))
+    got = oracle.canonical_dist_np(q, r)
+    assert abs(Fraction(got) - exact) <= 33 * 2.0 ** -52 * exact
+    acc = Fraction(0)
+    for a, b in zip(q.tolist(), r.tolist()):             # round after every step, exactly
+        t = Fraction(float(a) - float(b))
+        acc = Fraction(float(t * t + acc))
+    assert got == float(acc)
+    # a plain multiply-then-add (two roundings, the form this helper fell back to before) is a
+    # different expression: on random pairs it differs from the fma chain somewhere
+    differs = 0
+    for s in range(50):
+        q2, r2 = rng.random(33).astype(np.float32), rng.random(33).astype(np.float32)
+        plain = 0.0
+        for a, b in zip(q2.tolist(), r2.tolist()):
+            plain = plain + (a - b) * (a - b)
+        differs += plain != oracle.canonical_dist_np(q2, r2)
+    assert differs > 0
+    assert math.isfinite(got)
+
+
+def test_recall_at_k_hand_computed():
+    import oracle
+
+    true = np.array([[0, 1, 2, 3], [10, 11, 12, 13]])
+    assert oracle.recall_at_k(true.copy(), true) == 1.0
+    got = np.array([[3, 2, 9, 8], [13, 99, 98, 97]])           # 2/4 and 1/4 (order does not matter)
+    assert oracle.recall_at_k(got, true) == 3 / 8
+    dup = np.array([[0, 0, 0, 1], [-1, -1, 10, 11]])           # a repeat counts once; -1 never
+    assert oracle.recall_at_k(dup, true) == 4 / 8
+    wide = np.array([[7, 0, 1, 2, 3], [10, 11, 12, 99, 13]])   # only the first k columns count
+    assert oracle.recall_at_k(wide, true) == 6 / 8
+
+
+def test_max_rel_err_hand_computed():
+    import oracle
+
+    t = np.array([[1.0, 2.0], [4.0, 0.0]])
+    assert oracle.max_rel_err(t.copy(), t) == 0.0
+    g = np.array([[1.5, 2.0], [3.0, 0.0]])                     # 0.5 and 0.25 relative
+    assert oracle.max_rel_err(g, t) == 0.5
+    g0 = np.array([[1.0, 2.0], [4.0, 1e-300]])                 # a zero distance compares absolutely
+    assert oracle.max_rel_err(g0, t) == pytest.approx(1e-300 / np.finfo(np.float64).tiny)
+    assert oracle.max_rel_err(np.zeros((0, 3)), np.zeros((0, 3))) == 0.0
+    with pytest.raises(ValueError):
+        oracle.max_rel_err(np.zeros((2, 2)), np.zeros((2, 3)))
