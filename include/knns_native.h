@@ -95,6 +95,17 @@ knns_status knns_search(knns_corpus c, const void* queries, int dtype, long n,
 knns_status knns_search_multi(knns_corpus c, const void* queries, int dtype, long n,
                               const int* kq, const knns_params* p, void* dist, int32_t* idx);
 
+/* nb batches of independent requests packed into one search (one launch sequence): batch b has
+ * sizes[b] consecutive query rows, k = ks[b] (ks NULL: p->k for every batch) and, when self_rows
+ * is given and self_rows[b] >= 0, query j of batch b never returns corpus row self_rows[b] + j
+ * (all-kNN of a row range; -1: no exclusion for that batch). Outputs are packed per query as in
+ * knns_search_multi. No counterpart in the reference, whose C path searches one query set per
+ * call (src/cuknnBitonic.cu:422); it is the uneven-batch form of the all-kNN demo
+ * (demo/allknnTest.m:30-31), with self dropped by index. */
+knns_status knns_search_batches(knns_corpus c, const void* queries, int dtype, long nb,
+                                const long* sizes, const int* ks, const long* self_rows,
+                                const knns_params* p, void* dist, int32_t* idx);
+
 /* knns_search restricted to corpus rows [row_begin, row_end) (0 <= row_begin < row_end <= rows);
  * indices are corpus rows. No counterpart in the reference's exported API; it is the search of
  * one bucket of gpuknnLSH (cuknns.h:79) over a corpus uploaded once for all buckets. */

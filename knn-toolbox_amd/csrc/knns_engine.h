@@ -102,6 +102,7 @@ struct SearchLaunch {
   long self_offset = 0;
   bool out64 = false;
   int bitonic = 0;
+  bool self_preset = false;    // B.self already holds per-query self rows (the init kernel keeps them)
   const int* n_dev = nullptr;  // optional device query count <= n (graph-captured searches)
   Upload* feed = nullptr;      // optional: corpus rows still uploading; each scan waits for its rows
 };

@@ -526,7 +526,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     KNNS_TRY(launch_pack_fp8(pa, st));
   }
   KNNS_TRY(launch_init_queries((uint32_t*)B.thr, (int*)B.kc, (int*)B.kq, (long*)B.off,
-                               (int*)B.self, (int*)B.ccnt[0], n, n_pad, L.uniform ? L.kuni : 1,
+                               L.self_preset ? nullptr : (int*)B.self, (int*)B.ccnt[0], n, n_pad, L.uniform ? L.kuni : 1,
                                L.cmax, L.self_offset, m, L.n_dev, L.uniform ? 1 : 0, st));
   KNNS_TRY(launch_fill_i32((int*)B.flags, 0, 16, st));
 
@@ -625,9 +625,12 @@ bool is_device_ptr(const void* p) {
 }
 
 // Searches corpus rows [row_begin, row_end) (row_end < 0: to the last row); indices are corpus rows.
+// self_host (optional, n entries): corpus row query i never returns (< 0: none); it replaces
+// KNNS_EXCLUDE_SELF's self_offset + i.
 knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, const int* kq_host,
                       int kuni, const knns_params* p, void* dist, int32_t* idx,
-                      long row_begin = 0, long row_end = -1, Upload* feed = nullptr) {
+                      long row_begin = 0, long row_end = -1, Upload* feed = nullptr,
+                      const long* self_host = nullptr) {
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lock(E.mu);
   g_err.clear();
@@ -650,7 +653,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   if (prec == KNNS_PREC_FP8 && !c->x8)
     return fail(KNNS_ERR_INVALID, "FP8 precision needs a corpus built with KNNS_CORPUS_FP8");
   const bool dev = (p->flags & KNNS_DEVICE_PTRS) != 0;
-  const bool excl = (p->flags & KNNS_EXCLUDE_SELF) != 0;
+  const bool excl = (p->flags & KNNS_EXCLUDE_SELF) != 0 || self_host != nullptr;
   const bool out64 = c->dtype == KNNS_F64 || (p->flags & KNNS_DIST_F64);
   const bool ktimes = (p->flags & KNNS_KERNEL_TIMES) != 0;
   hipStream_t st = p->stream ? (hipStream_t)p->stream : E.stream;
@@ -746,20 +749,32 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     qsrc = stage;
   }
 #undef KNNS_WS
-  // per-query k: staged through pinned host memory (no synchronisation inside the search)
-  if (!uniform) {
+  // per-query k and self rows: staged through pinned host memory (no synchronisation inside the
+  // search). Layout: k and shortlist (4 bytes each per query), padded to 8 bytes, the offsets
+  // (8 bytes each), then the self rows of every query slot (4 bytes each, -1 past n).
+  if (!uniform || self_host) {
     void* pin;
-    // k and shortlist (4 bytes each per query), padded to 8 bytes, then the offsets (8 bytes each)
-    if ((sst = E.pinned(n * 16 + 8, &pin)) != KNNS_OK) return sst;
+    if ((sst = E.pinned(n * 16 + 8 + (long)P.n_pad * 4, &pin)) != KNNS_OK) return sst;
     int* pk = (int*)pin;
     int* pc = pk + n;
     long* po = (long*)(pc + n + (n & 1));
-    std::memcpy(pk, kq.data(), n * 4);
-    std::memcpy(pc, kc.data(), n * 4);
-    std::memcpy(po, off.data(), n * 8);
-    KNNS_HIP(hipMemcpyAsync(B.kq, pk, n * 4, hipMemcpyHostToDevice, st));
-    KNNS_HIP(hipMemcpyAsync(B.kc, pc, n * 4, hipMemcpyHostToDevice, st));
-    KNNS_HIP(hipMemcpyAsync(B.off, po, n * 8, hipMemcpyHostToDevice, st));
+    int* ps = (int*)(po + n);
+    if (!uniform) {
+      std::memcpy(pk, kq.data(), n * 4);
+      std::memcpy(pc, kc.data(), n * 4);
+      std::memcpy(po, off.data(), n * 8);
+      KNNS_HIP(hipMemcpyAsync(B.kq, pk, n * 4, hipMemcpyHostToDevice, st));
+      KNNS_HIP(hipMemcpyAsync(B.kc, pc, n * 4, hipMemcpyHostToDevice, st));
+      KNNS_HIP(hipMemcpyAsync(B.off, po, n * 8, hipMemcpyHostToDevice, st));
+    }
+    if (self_host) {
+      // self rows are searched-range rows: a row outside [row_begin, row_end) is never returned
+      for (long i = 0; i < P.n_pad; ++i) {
+        const long r = i < n ? self_host[i] : -1;
+        ps[i] = (r >= row_begin && r < row_end) ? (int)r : -1;
+      }
+      KNNS_HIP(hipMemcpyAsync(B.self, ps, (size_t)P.n_pad * 4, hipMemcpyHostToDevice, st));
+    }
   }
 
   SearchLaunch L;
@@ -773,6 +788,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   L.cmax = cmax;
   L.excl = excl;
   L.self_offset = p->self_offset;
+  L.self_preset = self_host != nullptr;
   L.out64 = out64;
   L.bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
   L.feed = feed;
@@ -950,6 +966,32 @@ knns_status knns_search_multi(knns_corpus c, const void* queries, int dtype, lon
                               const int* kq, const knns_params* p, void* dist, int32_t* idx) {
   if (!kq && n > 0) return fail(KNNS_ERR_INVALID, "null per-query k array");
   return do_search(c, queries, dtype, n, kq, 0, p, dist, idx);
+}
+
+knns_status knns_search_batches(knns_corpus c, const void* queries, int dtype, long nb,
+                                const long* sizes, const int* ks, const long* self_rows,
+                                const knns_params* p, void* dist, int32_t* idx) {
+  if (!p) return fail(KNNS_ERR_INVALID, "null params");
+  if (nb < 0) return fail(KNNS_ERR_INVALID, "negative batch count %ld", nb);
+  if (nb > 0 && !sizes) return fail(KNNS_ERR_INVALID, "null batch size array");
+  long n = 0;
+  for (long b = 0; b < nb; ++b) {
+    if (sizes[b] < 0) return fail(KNNS_ERR_INVALID, "batch %ld has %ld queries", b, sizes[b]);
+    n += sizes[b];
+    if (n > (1L << 30)) return fail(KNNS_ERR_INVALID, "too many queries");
+  }
+  std::vector<int> kq(n);
+  std::vector<long> self;
+  if (self_rows) self.resize(n);
+  long i = 0;
+  for (long b = 0; b < nb; ++b)
+    for (long j = 0; j < sizes[b]; ++j, ++i) {
+      kq[i] = ks ? ks[b] : p->k;
+      if (self_rows) self[i] = self_rows[b] < 0 ? -1 : self_rows[b] + j;
+    }
+  if (n == 0) return KNNS_OK;
+  return do_search(c, queries, dtype, n, kq.data(), 0, p, dist, idx, 0, -1, nullptr,
+                   self_rows ? self.data() : nullptr);
 }
 
 knns_status knns_search_rows(knns_corpus c, long row_begin, long row_end, const void* queries,

@@ -105,6 +105,8 @@ def native():
         L.knns_search.restype = I
         L.knns_search_multi.argtypes = [V, V, I, L_, V, ctypes.POINTER(Params), V, V]
         L.knns_search_multi.restype = I
+        L.knns_search_batches.argtypes = [V, V, I, L_, V, V, V, ctypes.POINTER(Params), V, V]
+        L.knns_search_batches.restype = I
         L.knns_search_rows.argtypes = [V, L_, L_, V, I, L_, ctypes.POINTER(Params), V, V]
         L.knns_search_rows.restype = I
         L.knns_search_once.argtypes = [V, I, L_, I, V, L_, ctypes.POINTER(Params), V, V]
@@ -309,6 +311,33 @@ class Corpus:
             _err(st)
         off = np.concatenate([[0], np.cumsum(kq)])
         return [(D[off[i]:off[i + 1]], I[off[i]:off[i + 1]]) for i in range(n)]
+
+
+    def search_batches(self, queries, sizes, ks=None, self_rows=None, precision: str = "fp32",
+                       select: str = "auto", shortlist: int = 0, k: int = 0):
+        """knns_search_batches: the batches (sizes[b] consecutive rows of `queries` each, k = ks[b]
+        or k, excluding corpus row self_rows[b] + j for query j when self_rows[b] >= 0) packed into
+        one search. Returns (dist, idx) as flat arrays packed per query, and the offsets."""
+        q = np.ascontiguousarray(queries, dtype=self.dtype)
+        sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+        nb = sizes.shape[0]
+        if int(sizes.sum()) != q.shape[0]:
+            raise KnnsError(KNNS_ERR_INVALID, f"batch sizes sum to {int(sizes.sum())}, not {q.shape[0]} queries")
+        kk = np.full(nb, int(k), dtype=np.int32) if ks is None else np.ascontiguousarray(ks, dtype=np.int32)
+        kq = np.repeat(kk, sizes)
+        tot = int(kq.sum())
+        out64 = self.dtype == np.float64
+        D = np.empty(tot, dtype=np.float64 if out64 else np.float32)
+        I = np.empty(tot, dtype=np.int32)
+        sr = None if self_rows is None else np.ascontiguousarray(self_rows, dtype=np.int64)
+        p = self._params(precision, int(k), select, shortlist, False, 0, 0, 0, 0)
+        st = native().knns_search_batches(self._h, q.ctypes.data, KNNS_F64 if out64 else KNNS_F32, nb,
+                                          sizes.ctypes.data, None if ks is None else kk.ctypes.data,
+                                          None if sr is None else sr.ctypes.data, ctypes.byref(p),
+                                          D.ctypes.data, I.ctypes.data)
+        if st:
+            _err(st)
+        return D, I, np.concatenate([[0], np.cumsum(kq)])
 
 
 # ------------------------------------------------------------------------------------------------

@@ -71,6 +71,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   mxArray* idx = mxCreateNumericMatrix((size_t)k, (size_t)q, KNNS_MX_CLASS, mxREAL);
   double t = 0.0;
   if (q > 0) {
+    // dTime / sTime (outputs 4 and 5) are per-stage device times: recorded only when asked for
+    knnsSetStageTimes(nlhs > 3 ? 1 : 0);
     t = KNNS_MEX_SEARCH((knntype*)mxGetData(Q), (knntype*)mxGetData(X), (knntype*)mxGetData(plhs[0]),
                         (knntype*)mxGetData(idx), (int)N, (int)q, (int)d, (int)k, (int)streams);
     const char* err = knnsGetLastError();

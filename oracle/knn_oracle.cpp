@@ -153,7 +153,26 @@ void exact_topk(const T* Q, const T* R, long n, long m, int d, int k, const int6
     for (long i = 0; i < n; ++i) {
       const T* q = Q + i * d;
       long cnt = 0;
-      for (long j = 0; j < m; ++j) {
+      // four rows at a time: four independent FMA chains, each in the canonical order (the
+      // results are those of canon_dist; only the latency of the chains overlaps)
+      long j = 0;
+      for (; j + 4 <= m; j += 4) {
+        const T* r0 = R + j * d;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int t = 0; t < d; ++t) {
+          const double qt = double(q[t]);
+          const double t0 = qt - double(r0[t]), t1 = qt - double(r0[d + t]);
+          const double t2 = qt - double(r0[2 * d + t]), t3 = qt - double(r0[3 * d + t]);
+          a0 = std::fma(t0, t0, a0);
+          a1 = std::fma(t1, t1, a1);
+          a2 = std::fma(t2, t2, a2);
+          a3 = std::fma(t3, t3, a3);
+        }
+        const double a[4] = {a0, a1, a2, a3};
+        for (int u = 0; u < 4; ++u)
+          if (!(self && self[i] == j + u)) buf[cnt++] = {a[u], int32_t(j + u)};
+      }
+      for (; j < m; ++j) {
         if (self && self[i] == j) continue;
         buf[cnt++] = {canon_dist(q, R + j * d, d), int32_t(j)};
       }
