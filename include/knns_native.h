@@ -112,6 +112,21 @@ knns_status knns_search_batches(knns_corpus c, const void* queries, int dtype, l
 knns_status knns_search_rows(knns_corpus c, long row_begin, long row_end, const void* queries,
                              int dtype, long n, const knns_params* p, void* dist, int32_t* idx);
 
+/* knns_search without any device or pinned-memory allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Queries and outputs are device pointers (p->flags must hold
+ * KNNS_DEVICE_PTRS), k is uniform (p->k), p->stream is used (NULL: the null stream), and the call
+ * returns after the search completed; KNNS_KERNEL_TIMES is rejected. The workspace comes from a
+ * fixed device array of KNNS_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until
+ * the process ends; the planner lowers the scan's slice count until the workspace fits (results
+ * are the same, only the scan's parallelism changes) and a search that does not fit even with one
+ * slice fails with KNNS_ERR_NOMEM. Calls in one process are serialised; the array keeps the last
+ * call's contents and is shared by every call on that GPU, so searches issued on one GPU from
+ * outside this serialisation (another copy of the library in the process) would race. */
+#define KNNS_NOALLOC_SCRATCH_BYTES (256L << 20)
+knns_status knns_search_noalloc(knns_corpus c, const void* queries, int dtype, long n,
+                                const knns_params* p, void* dist, int32_t* idx);
+
 /* knns_corpus_create + knns_search + knns_corpus_destroy in one call, for a corpus searched once
  * (the whole-problem call of gpuknnsBitonic / gpuknnsHeap, src/cuknnBitonic.cu:422-618): host rows
  * (m x d of `dtype`) upload on a second stream while the first threshold rounds, which scan a

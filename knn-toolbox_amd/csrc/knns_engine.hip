@@ -846,12 +846,164 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   return KNNS_OK;
 }
 
+// ---- search without allocation (knns_search_noalloc) ----
+// Every workspace buffer of the search is carved from `scratch`, a file-scope device array of
+// KNNS_NOALLOC_SCRATCH_BYTES that each GPU holds from the library's load until the process ends.
+// Its device address is looked up once per device (hipGetSymbolAddress, with that device
+// current) and kept in g_scratch_ptr: host code never passes the array's name to a kernel or a
+// copy, only that address.
+static __device__ __align__(256) unsigned char scratch[KNNS_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+int g_scratch_cus[kMaxDevices] = {};
+
+knns_status scratch_for_device(int dev, void** ptr, int* cus) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(KNNS_ERR_UNSUPPORTED, "device %d outside the first %d", dev, kMaxDevices);
+  if (!g_scratch_ptr[dev]) {
+    void* sp = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) {
+      (void)hipGetLastError();
+      return fail(KNNS_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(KNNS_ERR_DEVICE, "device query failed");
+    }
+    g_scratch_cus[dev] = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    g_scratch_ptr[dev] = sp;
+  }
+  *ptr = g_scratch_ptr[dev];
+  *cus = g_scratch_cus[dev];
+  return KNNS_OK;
+}
+
+// Bytes of the carved workspace (each buffer aligned to 256 bytes).
+size_t carve_bytes(const SearchSizes& z) {
+  const size_t parts[] = {z.q32, z.q16, z.q8, z.q64, z.thr, z.kc, z.kq, z.off, z.self, z.sub, z.subcnt,
+                          z.carry, z.carry, z.ccnt, z.ccnt, z.flags};
+  size_t t = 0;
+  for (size_t b : parts) t += (b + 255) & ~(size_t)255;
+  return t;
+}
+
+knns_status do_search_noalloc(knns_corpus c, const void* queries, int dtype, long n, const knns_params* p,
+                              void* dist, int32_t* idx) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lock(E.mu);   // calls of one process never overlap
+  g_err.clear();
+  if (!c) return fail(KNNS_ERR_INVALID, "null corpus");
+  if (!p) return fail(KNNS_ERR_INVALID, "null params");
+  if (n < 0) return fail(KNNS_ERR_INVALID, "negative query count %ld", n);
+  if (n == 0) return KNNS_OK;
+  if (!queries || !dist || !idx) return fail(KNNS_ERR_INVALID, "null query/output pointer");
+  if (!(p->flags & KNNS_DEVICE_PTRS))
+    return fail(KNNS_ERR_INVALID, "knns_search_noalloc takes device pointers (KNNS_DEVICE_PTRS)");
+  if (p->flags & KNNS_KERNEL_TIMES) return fail(KNNS_ERR_INVALID, "knns_search_noalloc records no kernel times");
+  if (dtype != c->dtype)
+    return fail(KNNS_ERR_INVALID, "query dtype %d does not match corpus dtype %d", dtype, c->dtype);
+  if (n > (1L << 30)) return fail(KNNS_ERR_INVALID, "too many queries (%ld)", n);
+  const int prec = p->precision;
+  if (prec != KNNS_PREC_FP32 && prec != KNNS_PREC_FP8 && prec != KNNS_PREC_BF16 && prec != KNNS_PREC_FP64)
+    return fail(KNNS_ERR_INVALID, "unknown precision %d", prec);
+  if (prec == KNNS_PREC_FP64 && c->dtype != KNNS_F64) return fail(KNNS_ERR_INVALID, "FP64 precision needs a double corpus");
+  if (prec == KNNS_PREC_BF16 && !c->x16)
+    return fail(KNNS_ERR_INVALID, "BF16 precision needs a corpus built with KNNS_CORPUS_BF16");
+  if (prec == KNNS_PREC_FP8 && !c->x8) return fail(KNNS_ERR_INVALID, "FP8 precision needs a corpus built with KNNS_CORPUS_FP8");
+  const int k = p->k;
+  if (k < 1 || k > 1024) return fail(KNNS_ERR_INVALID, "k=%d outside [1, 1024]", k);
+  const bool excl = (p->flags & KNNS_EXCLUDE_SELF) != 0;
+  const bool out64 = c->dtype == KNNS_F64 || (p->flags & KNNS_DIST_F64);
+  int dev = 0;
+  KNNS_HIP(hipGetDevice(&dev));
+  void* base = nullptr;
+  int cus = 256;
+  if (knns_status st = scratch_for_device(dev, &base, &cus)) return st;
+  const long m = c->m;
+  const long m_eff = excl ? std::max<long>(m - 1, 1) : m;
+  int cmax = p->shortlist > 0 ? std::max(p->shortlist, k) : default_shortlist(prec, k);
+  cmax = std::max(std::min((int)std::min<long>(cmax, m_eff), 2048), 1);
+  // the planner's slice count, lowered until the workspace fits the scratch array (the answer does
+  // not depend on the slice count, only the parallelism of the scan does)
+  const Tuning tu = tuning_for(prec, c->d, k, n);
+  int sreq = p->slices;
+  Plan P;
+  SearchSizes z;
+  for (;;) {
+    P = make_plan(m, c->d, n, k, cmax, prec, cus, p->ratio > 0 ? p->ratio : tu.ratio, sreq, tu.min_waves);
+    z = search_sizes(c, P, n, n * (long)k, excl, out64);
+    if (carve_bytes(z) <= (size_t)KNNS_NOALLOC_SCRATCH_BYTES) break;
+    if (P.slices_max <= 1)
+      return fail(KNNS_ERR_NOMEM,
+                  "search needs %zu bytes of workspace even with one slice (%ld queries, k=%d): above the %ld-byte "
+                  "scratch of knns_search_noalloc; use knns_search or fewer queries per call",
+                  carve_bytes(z), n, k, (long)KNNS_NOALLOC_SCRATCH_BYTES);
+    sreq = std::max(1, P.slices_max / 2);
+  }
+  if (P.dp != c->dp) return fail(KNNS_ERR_INVALID, "corpus/plan dimension mismatch");
+  SearchBuffers B;
+  char* cur = (char*)base;
+  auto take = [&](size_t bytes) -> void* {
+    if (!bytes) return nullptr;
+    void* r = cur;
+    cur += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  B.q32 = take(z.q32);
+  B.q16 = take(z.q16);
+  B.q8 = take(z.q8);
+  B.q64 = take(z.q64);
+  B.thr = take(z.thr);
+  B.kc = take(z.kc);
+  B.kq = take(z.kq);
+  B.off = take(z.off);
+  B.self = take(z.self);
+  B.sub = take(z.sub);
+  B.subcnt = take(z.subcnt);
+  B.carry[0] = take(z.carry);
+  B.carry[1] = take(z.carry);
+  B.ccnt[0] = take(z.ccnt);
+  B.ccnt[1] = take(z.ccnt);
+  B.flags = take(z.flags);
+  B.odist = dist;
+  B.oidx = idx;
+  hipStream_t st = (hipStream_t)p->stream;   // NULL: the device's null stream
+  SearchLaunch L;
+  L.c = c;
+  L.P = &P;
+  L.n = n;
+  L.dtype = dtype;
+  L.qsrc = queries;
+  L.uniform = true;
+  L.kuni = k;
+  L.cmax = cmax;
+  L.excl = excl;
+  L.self_offset = p->self_offset;
+  L.out64 = out64;
+  L.bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
+  // the flags (overflow counter) are cleared by a fill kernel inside the sequence
+  KNNS_HIP(enqueue_search(L, B, st, nullptr));
+  int flags_h[16] = {0};
+  KNNS_HIP(hipMemcpyAsync(flags_h, B.flags, sizeof(flags_h), hipMemcpyDeviceToHost, st));
+  KNNS_HIP(hipStreamSynchronize(st));
+  if (flags_h[0] != 0)
+    return fail(KNNS_ERR_OVERFLOW, "candidate pool overflow (%d events): too many exact ties", flags_h[0]);
+  return KNNS_OK;
+}
+
 }  // namespace
 }  // namespace knns
 
 using namespace knns;
 
 extern "C" {
+
+knns_status knns_search_noalloc(knns_corpus c, const void* queries, int dtype, long n,
+                                const knns_params* p, void* dist, int32_t* idx) {
+  return do_search_noalloc(c, queries, dtype, n, p, dist, idx);
+}
 
 void knns_params_init(knns_params* p, int precision, int k) {
   if (!p) return;

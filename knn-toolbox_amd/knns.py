@@ -107,6 +107,8 @@ def native():
         L.knns_search_multi.restype = I
         L.knns_search_batches.argtypes = [V, V, I, L_, V, V, V, ctypes.POINTER(Params), V, V]
         L.knns_search_batches.restype = I
+        L.knns_search_noalloc.argtypes = [V, V, I, L_, ctypes.POINTER(Params), V, V]
+        L.knns_search_noalloc.restype = I
         L.knns_search_rows.argtypes = [V, L_, L_, V, I, L_, ctypes.POINTER(Params), V, V]
         L.knns_search_rows.restype = I
         L.knns_search_once.argtypes = [V, I, L_, I, V, L_, ctypes.POINTER(Params), V, V]
@@ -292,6 +294,19 @@ class Corpus:
                                   KNNS_F64 if self.dtype == np.float64 else KNNS_F32, n, ctypes.byref(p),
                                   _ptr(dist_ptr) if not isinstance(dist_ptr, int) else dist_ptr,
                                   _ptr(idx_ptr) if not isinstance(idx_ptr, int) else idx_ptr)
+        if st:
+            _err(st)
+
+    def search_noalloc(self, q_ptr, n: int, k: int, dist_ptr, idx_ptr, precision: str = "fp32",
+                       select: str = "auto", shortlist: int = 0, stream: int = 0):
+        """knns_search_noalloc: device pointers (ints or torch tensors), no allocation; the
+        workspace is the library's fixed device scratch array. Synchronous."""
+        p = self._params(precision, k, select, shortlist, False, 0, 0, 0, KNNS_DEVICE_PTRS)
+        p.stream = stream or None
+        st = native().knns_search_noalloc(self._h, q_ptr if isinstance(q_ptr, int) else _ptr(q_ptr),
+                                          KNNS_F64 if self.dtype == np.float64 else KNNS_F32, n, ctypes.byref(p),
+                                          dist_ptr if isinstance(dist_ptr, int) else _ptr(dist_ptr),
+                                          idx_ptr if isinstance(idx_ptr, int) else _ptr(idx_ptr))
         if st:
             _err(st)
 
