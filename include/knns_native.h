@@ -52,6 +52,9 @@ typedef enum {
 #define KNNS_DIST_F64 0x4u      /* write double distances (default: float, or double for FP64) */
 #define KNNS_KERNEL_TIMES 0x8u  /* record per-kernel device times (knns_stats scan_s / select_s); adds
                                    an event per launch, so leave it off when timing whole searches */
+#define KNNS_NO_CERTIFY 0x10u   /* skip the shortlist certificate and the re-search of uncertified
+                                   queries: the top-k is then exact only when no true neighbour ranks
+                                   past the shortlist by approximate key (diagnostics) */
 
 typedef struct knns_corpus_s* knns_corpus;
 
@@ -75,6 +78,13 @@ typedef struct {
   long slow_compactions;
   long candidates;   /* total entries appended by the scan kernels */
   int groups, slices_max, qb, dc, dp;
+  /* Shortlist certificate: a query is certified when every corpus row outside its shortlist
+   * provably lies farther than its k-th exact distance (bounds: operand rounding radii and the
+   * FP32 accumulation error of the approximate keys). Uncertified queries of the exact forms
+   * (FP32, BF16, FP64) are searched again with every row inside the bound admitted, so their
+   * results are exact by construction; FP8 only counts them. */
+  long uncertified;  /* queries whose first shortlist was not certified */
+  long researched;   /* queries searched again */
 } knns_stats;
 
 /* fills *p with the defaults of a precision */

@@ -20,6 +20,7 @@ struct knns_corpus_s {
   float* x32 = nullptr;
   uint16_t* x16 = nullptr;
   double* x64 = nullptr;
+  uint32_t* bnd = nullptr; // certificate maxima per operand copy (PackArgs::bnd), BND_SLOTS words
   uint8_t* x8 = nullptr;   // FP8 (E4M3) rows with their scale, rows_pad x ld8 bytes
   int dp8 = 0;             // FP8 row bytes (d rounded up to 64, 128 or 256)
   long ld8 = 0;            // dp8 + 16 (the augmentation unit)
@@ -58,7 +59,12 @@ struct Tuning {
 Tuning tuning_for(int prec, int d, int k, long n);
 
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
-               int slices_req, int min_waves = 5);
+               int slices_req, int min_waves = 5, bool allow_dense = true);
+
+// Relative error bound of the scan's FP32 accumulation of one key (every rounding of it), for
+// the certificate: gamma = n u / (1 - n u), n = roundings per key, u = 2^-23 (covers rounding
+// modes other than to-nearest), plus the representation error of the norm terms.
+double key_gamma(const Plan& P);
 
 // Device buffers of one search.
 struct SearchBuffers {
@@ -75,14 +81,16 @@ struct SearchBuffers {
   void* subcnt = nullptr;   // sub-pool fills
   void* carry[2] = {nullptr, nullptr};
   void* ccnt[2] = {nullptr, nullptr};
-  void* flags = nullptr;    // 16 ints
+  void* flags = nullptr;    // 16 ints: [0] overflow events, [1] slow compactions, [2] uncertified queries
+  void* qn2 = nullptr;      // ||q~||^2 per query slot (double)
+  void* qrho = nullptr;     // ||q~ - q|| per query slot (double)
   void* odist = nullptr;    // output distances
   void* oidx = nullptr;     // output indices
 };
 
 // Byte sizes of the buffers above for a plan (q_slots = query slots of the search).
 struct SearchSizes {
-  size_t q32, q16, q8, q64, thr, kc, kq, off, self, sub, subcnt, carry, ccnt, flags, odist, oidx;
+  size_t q32, q16, q8, q64, thr, kc, kq, off, self, sub, subcnt, carry, ccnt, flags, odist, oidx, qn2, qrho;
 };
 SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long total_out, bool excl,
                          bool out64);
@@ -103,9 +111,43 @@ struct SearchLaunch {
   bool out64 = false;
   int bitonic = 0;
   bool self_preset = false;    // B.self already holds per-query self rows (the init kernel keeps them)
+  bool thr_preset = false;     // B.thr already holds the real queries' initial thresholds
+  bool certify = true;         // the final kernel checks each query's shortlist certificate
   const int* n_dev = nullptr;  // optional device query count <= n (graph-captured searches)
   Upload* feed = nullptr;      // optional: corpus rows still uploading; each scan waits for its rows
 };
+
+// Device workspace of a re-search: buffers by slot (the engine's grow-only slots, or a fixed
+// array carved in order) and a host-to-device copy of a control block.
+struct Workspace {
+  virtual ~Workspace() {}
+  virtual size_t budget() const = 0;   // bytes the re-search may use
+  virtual knns_status get(int slot, size_t bytes, void** out) = 0;
+  // enqueues a copy of `bytes` from host memory valid until the caller synchronises `st`
+  virtual knns_status upload(void* dst, const void* src, size_t bytes, hipStream_t st) = 0;
+};
+
+// Uncertified queries of a finished search (the final kernel left their needed thresholds).
+struct ResearchIn {
+  const knns_corpus_s* c = nullptr;
+  int prec = KNNS_PREC_FP32;
+  int dtype = KNNS_F32;
+  bool out64 = false, excl = false, bitonic = false;
+  long row_begin = 0, row_end = 0;
+  const void* qsrc = nullptr;          // device query rows of the search (dtype, d per row)
+  std::vector<int> qids;               // query indices to re-search (rows of qsrc)
+  std::vector<uint32_t> thr;           // their thresholds (ord): every row below is admitted
+  std::vector<int> kq;                 // their k
+  std::vector<long> off;               // their output offsets (entries)
+  std::vector<int> self;               // their excluded rows (-1: none; used when excl)
+  void* odist = nullptr;               // the search's device outputs, rewritten for these queries
+  int32_t* oidx = nullptr;
+};
+
+// Re-searches R.qids with every row below their threshold admitted (shortlists of up to 2048),
+// writes their k results at their offsets and synchronises `st`. Fails when a re-search still
+// cannot be certified.
+knns_status run_research(const ResearchIn& R, Workspace& ws, hipStream_t st, int cus, long* researched);
 
 // Timing hook: called with 0/1 around each scan launch, 2 after each merge, 3/4 around the final
 // kernel (used to record events; nullptr records nothing).

@@ -46,7 +46,7 @@ namespace {
 
 enum Slot {
   W_Q32, W_Q16, W_Q8, W_Q64, W_STAGE, W_THR, W_KC, W_KQ, W_OFF, W_SELF, W_SUB, W_SUBCNT, W_CARRY0,
-  W_CARRY1, W_CCNT0, W_CCNT1, W_FLAGS, W_ODIST, W_OIDX, W_NSLOTS
+  W_CARRY1, W_CCNT0, W_CCNT1, W_FLAGS, W_ODIST, W_OIDX, W_QN2, W_QRHO, W_RQ, W_RCTL, W_NSLOTS
 };
 
 struct Engine {
@@ -268,7 +268,7 @@ int default_shortlist(int prec, int k) {
 }
 
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
-               int slices_req, int min_waves) {
+               int slices_req, int min_waves, bool allow_dense) {
   Plan P{};
   dims_for(d, &P.dp, &P.dc);
   P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : (prec == KNNS_PREC_FP8 ? PREC_FP8 : PREC_F32);
@@ -321,7 +321,7 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
     int s = (int)std::min<long>(smax, std::max<long>(1, rows / gr));
     r.slice_rows = round_up((rows + s - 1) / s, gr);
     r.slices = (int)((rows + r.slice_rows - 1) / r.slice_rows);
-    r.dense = (b == 0 && r.slice_rows <= P.sub_cap) ? 1 : 0;
+    r.dense = (allow_dense && b == 0 && r.slice_rows <= P.sub_cap) ? 1 : 0;
     smax_used = std::max(smax_used, r.slices);
     P.rounds.push_back(r);
     b = e;
@@ -330,6 +330,16 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   }
   P.slices_max = smax_used;
   return P;
+}
+
+double key_gamma(const Plan& P) {
+  // roundings along one key's accumulation: FP32 products are rounded (two per k-step of the
+  // 32x32x2 MFMA, plus the hi/lo norm step); BF16 and FP8 products are exact in FP32 and each
+  // k-step adds its products to the accumulator; the norm step adds up to 8 terms
+  const double n = P.prec_kernel == PREC_F32 ? 2.0 * (P.dp + 4) : (double)(P.prec_kernel == PREC_FP8 ? P.dc : P.dp) + 16.0;
+  const double u = 0x1p-23;
+  // + the norm's split into 3 BF16 terms (relative 2^-27) or 2 FP32 terms
+  return n * u / (1.0 - n * u) + 0x1p-26;
 }
 
 SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long total_out, bool excl,
@@ -352,6 +362,8 @@ SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long tot
   z.flags = 16 * 4;
   z.odist = (size_t)total_out * (out64 ? 8 : 4);
   z.oidx = (size_t)total_out * 4;
+  z.qn2 = (size_t)n_pad * 8;
+  z.qrho = (size_t)n_pad * 8;
   return z;
 }
 
@@ -406,6 +418,8 @@ struct Upload {
       ready = nullptr;
       return fail(KNNS_ERR_DEVICE, "hipEventCreate failed");
     }
+    // the certificate maxima start at zero, before the first pack raises them
+    if (launch_fill_u32(c->bnd, 0u, BND_SLOTS, st) != hipSuccess) return fail(KNNS_ERR_DEVICE, "bounds clear failed");
     if (dev) return KNNS_OK;
     if (hipMalloc(&stage, (size_t)chunk * row_bytes) != hipSuccess) {
       (void)hipGetLastError();
@@ -461,6 +475,7 @@ struct Upload {
     pa.x8 = c->x8 ? c->x8 + (size_t)r0 * c->ld8 : nullptr;
     pa.ld8 = c->ld8;
     pa.dp8 = c->dp8;
+    pa.bnd = c->bnd;
     if (launch_pack(pa, st) != hipSuccess || (c->x8 && launch_pack_fp8(pa, st) != hipSuccess))
       return fail(KNNS_ERR_DEVICE, "corpus pack launch failed");
     if (hipEventRecord(ready, st) != hipSuccess) return fail(KNNS_ERR_DEVICE, "hipEventRecord failed");
@@ -518,6 +533,9 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   pa.x64 = (double*)B.q64;
   pa.dp = c->dp;
   pa.is_query = 1;
+  pa.qn2 = (double*)B.qn2;
+  pa.qrho = (double*)B.qrho;
+  pa.qprec = P.prec_kernel;
   KNNS_TRY(launch_pack(pa, st));
   if (P.prec_kernel == PREC_FP8) {
     pa.x8 = (uint8_t*)B.q8;
@@ -527,7 +545,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   }
   KNNS_TRY(launch_init_queries((uint32_t*)B.thr, (int*)B.kc, (int*)B.kq, (long*)B.off,
                                L.self_preset ? nullptr : (int*)B.self, (int*)B.ccnt[0], n, n_pad, L.uniform ? L.kuni : 1,
-                               L.cmax, L.self_offset, m, L.n_dev, L.uniform ? 1 : 0, st));
+                               L.cmax, L.self_offset, m, L.n_dev, L.uniform ? 1 : 0, L.thr_preset ? 0 : 1, st));
   KNNS_TRY(launch_fill_i32((int*)B.flags, 0, 16, st));
 
   const int pk = P.prec_kernel;
@@ -605,6 +623,14 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   fa.out_idx = (int32_t*)B.oidx;
   fa.n = (int)n;
   fa.n_dev = L.n_dev;
+  fa.certify = L.certify && B.qn2 ? 1 : 0;
+  fa.thr = (uint32_t*)B.thr;
+  fa.flags = (int*)B.flags;
+  fa.qn2 = (const double*)B.qn2;
+  fa.qrho = (const double*)B.qrho;
+  fa.bnd = c->bnd;
+  fa.bslot = P.prec_kernel;
+  fa.gamma = key_gamma(P);
   if (L.feed) KNNS_TRY(feed_rows(*L.feed, c->rows_pad, st));
   if (hook) (*hook)(3);
   KNNS_TRY(launch_final(fa, st));
@@ -623,6 +649,140 @@ bool is_device_ptr(const void* p) {
   }
   return at.type == hipMemoryTypeDevice;
 }
+
+}  // namespace
+
+// ---- re-search of uncertified queries ----
+knns_status run_research(const ResearchIn& R, Workspace& ws, hipStream_t st, int cus, long* researched) {
+  const knns_corpus_s* c = R.c;
+  const long nf = (long)R.qids.size();
+  *researched = nf;
+  if (nf == 0) return KNNS_OK;
+  const long m = R.row_end - R.row_begin;
+  const long m_eff = R.excl ? std::max<long>(m - 1, 1) : m;
+  const int cap = (int)std::max<long>(1, std::min<long>(2048, m_eff));
+  int kmax = 1;
+  for (int k : R.kq) kmax = std::max(kmax, k);
+  const size_t qsz = R.dtype == KNNS_F64 ? 8 : 4;
+  const size_t row_bytes = (size_t)c->d * qsz;
+  // plan: the preset thresholds replace the dense first round; the slice count is lowered until
+  // the workspace fits the provider's budget (the answer does not depend on it)
+  const Tuning tu = tuning_for(R.prec, c->d, kmax, nf);
+  int sreq = 0;
+  Plan P;
+  SearchSizes z;
+  size_t ctl_bytes = 0, total = 0;
+  long n_pad = 0;
+  for (;;) {
+    // few rounds: the thresholds start where the certificate needs them and rarely tighten
+    P = make_plan(m, c->d, nf, kmax, cap, R.prec, cus, 64, sreq, tu.min_waves, false);
+    z = search_sizes(c, P, nf, 0, R.excl, R.out64);
+    n_pad = P.n_pad;
+    ctl_bytes = (size_t)nf * (4 + 4 + 8 + 4) + (size_t)n_pad * (4 + 4) + 64;
+    total = 0;
+    const size_t parts[] = {z.q32, z.q16, z.q8, z.q64, z.sub, z.subcnt, z.carry, z.carry, z.ccnt, z.ccnt,
+                            z.flags, z.qn2, z.qrho, (size_t)nf * row_bytes, ctl_bytes};
+    for (size_t b : parts) total += (b + 255) & ~(size_t)255;
+    if (total <= ws.budget()) break;
+    if (P.slices_max <= 1)
+      return fail(KNNS_ERR_NOMEM, "re-search of %ld uncertified queries needs %zu bytes of workspace (budget %zu)",
+                  nf, total, ws.budget());
+    sreq = std::max(1, P.slices_max / 2);
+  }
+  for (Round& r : P.rounds) {
+    r.begin += R.row_begin;
+    r.end += R.row_begin;
+  }
+  // control block: k, shortlist, output offset, source row of each query; threshold and self row of
+  // each query slot
+  std::vector<char> ctl(ctl_bytes, 0);
+  size_t o = 0;
+  auto place = [&](size_t bytes) { const size_t r = o; o += (bytes + 15) & ~(size_t)15; return r; };
+  const size_t o_kq = place(nf * 4), o_kc = place(nf * 4), o_off = place(nf * 8), o_idx = place(nf * 4),
+               o_thr = place(n_pad * 4), o_self = place(n_pad * 4);
+  if (o > ctl_bytes) return fail(KNNS_ERR_INVALID, "internal: control block layout");
+  for (long i = 0; i < nf; ++i) {
+    ((int*)(ctl.data() + o_kq))[i] = R.kq[i];
+    ((int*)(ctl.data() + o_kc))[i] = cap;
+    ((long*)(ctl.data() + o_off))[i] = R.off[i];
+    ((int*)(ctl.data() + o_idx))[i] = R.qids[i];
+    ((uint32_t*)(ctl.data() + o_thr))[i] = R.thr[i];
+  }
+  for (long i = 0; i < n_pad; ++i) ((int*)(ctl.data() + o_self))[i] = (R.excl && i < nf) ? R.self[i] : -1;
+  SearchBuffers B;
+  void* rq = nullptr;
+  void* dctl = nullptr;
+  knns_status sst;
+#define KNNS_RWS(slot, bytes, ptr) \
+  if ((bytes) && (sst = ws.get(slot, (size_t)(bytes), &ptr)) != KNNS_OK) return sst
+  KNNS_RWS(W_Q32, z.q32, B.q32);
+  KNNS_RWS(W_Q16, z.q16, B.q16);
+  KNNS_RWS(W_Q8, z.q8, B.q8);
+  KNNS_RWS(W_Q64, z.q64, B.q64);
+  KNNS_RWS(W_SUB, z.sub, B.sub);
+  KNNS_RWS(W_SUBCNT, z.subcnt, B.subcnt);
+  KNNS_RWS(W_CARRY0, z.carry, B.carry[0]);
+  KNNS_RWS(W_CARRY1, z.carry, B.carry[1]);
+  KNNS_RWS(W_CCNT0, z.ccnt, B.ccnt[0]);
+  KNNS_RWS(W_CCNT1, z.ccnt, B.ccnt[1]);
+  KNNS_RWS(W_FLAGS, z.flags, B.flags);
+  KNNS_RWS(W_QN2, z.qn2, B.qn2);
+  KNNS_RWS(W_QRHO, z.qrho, B.qrho);
+  KNNS_RWS(W_RQ, (size_t)nf * row_bytes, rq);
+  KNNS_RWS(W_RCTL, ctl_bytes, dctl);
+#undef KNNS_RWS
+  B.kq = (char*)dctl + o_kq;
+  B.kc = (char*)dctl + o_kc;
+  B.off = (char*)dctl + o_off;
+  B.thr = (char*)dctl + o_thr;
+  B.self = R.excl ? (char*)dctl + o_self : nullptr;
+  B.odist = R.odist;
+  B.oidx = R.oidx;
+  if ((sst = ws.upload(dctl, ctl.data(), ctl_bytes, st)) != KNNS_OK) return sst;
+  KNNS_HIP(launch_gather_rows(rq, R.qsrc, (const int*)((char*)dctl + o_idx), nf, (long)row_bytes, st));
+  SearchLaunch L;
+  L.c = c;
+  L.P = &P;
+  L.n = nf;
+  L.dtype = R.dtype;
+  L.qsrc = rq;
+  L.uniform = false;
+  L.kuni = 1;
+  L.cmax = cap;
+  L.excl = R.excl;
+  L.self_preset = true;
+  L.thr_preset = true;
+  L.certify = true;
+  L.out64 = R.out64;
+  L.bitonic = R.bitonic ? 1 : 0;
+  KNNS_HIP(enqueue_search(L, B, st, nullptr));
+  int flags_h[16] = {0};
+  KNNS_HIP(hipMemcpyAsync(flags_h, B.flags, sizeof(flags_h), hipMemcpyDeviceToHost, st));
+  KNNS_HIP(hipStreamSynchronize(st));
+  if (flags_h[0] != 0)
+    return fail(KNNS_ERR_OVERFLOW, "candidate pool overflow in the re-search (%d events): too many exact ties", flags_h[0]);
+  if (flags_h[2] != 0)
+    return fail(KNNS_ERR_OVERFLOW, "%d queries cannot be certified: more than %d corpus rows lie within the "
+                "distance error bound of their k-th neighbour", flags_h[2], cap);
+  return KNNS_OK;
+}
+
+namespace {
+
+// The engine's own workspace for a re-search: its grow-only slots and pinned staging.
+struct EngineWorkspace : Workspace {
+  Engine& E;
+  explicit EngineWorkspace(Engine& e) : E(e) {}
+  size_t budget() const override { return (size_t)4 << 30; }
+  knns_status get(int slot, size_t bytes, void** out) override { return E.get((Slot)slot, bytes, out); }
+  knns_status upload(void* dst, const void* src, size_t bytes, hipStream_t st) override {
+    void* pin = nullptr;
+    if (knns_status s = E.pinned(bytes, &pin)) return s;
+    std::memcpy(pin, src, bytes);
+    KNNS_HIP(hipMemcpyAsync(dst, pin, bytes, hipMemcpyHostToDevice, st));
+    return KNNS_OK;
+  }
+};
 
 // Searches corpus rows [row_begin, row_end) (row_end < 0: to the last row); indices are corpus rows.
 // self_host (optional, n entries): corpus row query i never returns (< 0: none); it replaces
@@ -727,6 +887,8 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   KNNS_WS(W_CCNT0, z.ccnt, B.ccnt[0]);
   KNNS_WS(W_CCNT1, z.ccnt, B.ccnt[1]);
   KNNS_WS(W_FLAGS, z.flags, B.flags);
+  KNNS_WS(W_QN2, z.qn2, B.qn2);
+  KNNS_WS(W_QRHO, z.qrho, B.qrho);
   if (!dev) {
     KNNS_WS(W_ODIST, z.odist, B.odist);
     KNNS_WS(W_OIDX, z.oidx, B.oidx);
@@ -792,6 +954,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   L.out64 = out64;
   L.bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
   L.feed = feed;
+  L.certify = (p->flags & KNNS_NO_CERTIFY) == 0;
   // per-launch events (KNNS_KERNEL_TIMES): pairs (scan begin, scan end), (scan end, merge end),
   // (final begin, final end)
   std::vector<std::pair<size_t, size_t>> scan_ev, sel_ev;
@@ -816,9 +979,50 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     KNNS_HIP(hipMemcpyAsync(dist, B.odist, z.odist, hipMemcpyDeviceToHost, st));
     KNNS_HIP(hipMemcpyAsync(idx, B.oidx, z.oidx, hipMemcpyDeviceToHost, st));
   }
-  const size_t e_end = ev++;
+  size_t e_end = ev++;
   KNNS_HIP(hipEventRecord(E.event(e_end), st));
   KNNS_HIP(hipStreamSynchronize(st));
+
+  // Queries whose shortlist the final kernel could not certify are searched again with every row
+  // below their certificate threshold admitted (exact forms; FP8 is the approximate form and only
+  // reports them).
+  long researched = 0;
+  if (flags_h[2] > 0 && flags_h[0] == 0 && L.certify && prec != KNNS_PREC_FP8) {
+    ResearchIn R;
+    R.c = c;
+    R.prec = prec;
+    R.dtype = dtype;
+    R.out64 = out64;
+    R.excl = excl;
+    R.bitonic = L.bitonic != 0;
+    R.row_begin = row_begin;
+    R.row_end = row_end;
+    R.qsrc = qsrc;
+    R.odist = B.odist;
+    R.oidx = (int32_t*)B.oidx;
+    std::vector<uint32_t> need(n);
+    KNNS_HIP(hipMemcpy(need.data(), B.thr, n * 4, hipMemcpyDeviceToHost));
+    for (long i = 0; i < n; ++i) {
+      if (!need[i]) continue;
+      R.qids.push_back((int)i);
+      R.thr.push_back(need[i]);
+      R.kq.push_back(uniform ? kuni : kq[i]);
+      R.off.push_back(uniform ? i * (long)kuni : off[i]);
+      long sr = -1;
+      if (self_host) sr = self_host[i];
+      else if (excl) sr = p->self_offset + i;
+      R.self.push_back((sr >= row_begin && sr < row_end && sr < c->m) ? (int)sr : -1);
+    }
+    EngineWorkspace ws(E);
+    if (knns_status rs = run_research(R, ws, st, E.cus, &researched)) return rs;
+    if (!dev) {
+      KNNS_HIP(hipMemcpyAsync(dist, B.odist, z.odist, hipMemcpyDeviceToHost, st));
+      KNNS_HIP(hipMemcpyAsync(idx, B.oidx, z.oidx, hipMemcpyDeviceToHost, st));
+    }
+    e_end = ev++;
+    KNNS_HIP(hipEventRecord(E.event(e_end), st));
+    KNNS_HIP(hipStreamSynchronize(st));
+  }
 
   knns_stats S{};
   float ms = 0.f;
@@ -839,6 +1043,8 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   S.qb = P.qb;
   S.dc = P.dc;
   S.dp = P.dp;
+  S.uncertified = flags_h[2];
+  S.researched = researched;
   E.last = S;
   if (flags_h[0] != 0)
     return fail(KNNS_ERR_OVERFLOW, "candidate pool overflow (%d events): too many exact ties",
@@ -880,10 +1086,30 @@ knns_status scratch_for_device(int dev, void** ptr, int* cus) {
   return KNNS_OK;
 }
 
+// The scratch array as a re-search workspace: buffers carved in order, copies from pageable host
+// memory (no pinned staging).
+struct ScratchWorkspace : Workspace {
+  char* base;
+  size_t cap, used = 0;
+  ScratchWorkspace(void* b, size_t c) : base((char*)b), cap(c) {}
+  size_t budget() const override { return cap; }
+  knns_status get(int, size_t bytes, void** out) override {
+    const size_t b = (bytes + 255) & ~(size_t)255;
+    if (used + b > cap) return fail(KNNS_ERR_NOMEM, "re-search workspace exceeds the scratch array");
+    *out = base + used;
+    used += b;
+    return KNNS_OK;
+  }
+  knns_status upload(void* dst, const void* src, size_t bytes, hipStream_t st) override {
+    KNNS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    return KNNS_OK;
+  }
+};
+
 // Bytes of the carved workspace (each buffer aligned to 256 bytes).
 size_t carve_bytes(const SearchSizes& z) {
   const size_t parts[] = {z.q32, z.q16, z.q8, z.q64, z.thr, z.kc, z.kq, z.off, z.self, z.sub, z.subcnt,
-                          z.carry, z.carry, z.ccnt, z.ccnt, z.flags};
+                          z.carry, z.carry, z.ccnt, z.ccnt, z.flags, z.qn2, z.qrho};
   size_t t = 0;
   for (size_t b : parts) t += (b + 255) & ~(size_t)255;
   return t;
@@ -967,6 +1193,8 @@ knns_status do_search_noalloc(knns_corpus c, const void* queries, int dtype, lon
   B.ccnt[0] = take(z.ccnt);
   B.ccnt[1] = take(z.ccnt);
   B.flags = take(z.flags);
+  B.qn2 = take(z.qn2);
+  B.qrho = take(z.qrho);
   B.odist = dist;
   B.oidx = idx;
   hipStream_t st = (hipStream_t)p->stream;   // NULL: the device's null stream
@@ -983,6 +1211,7 @@ knns_status do_search_noalloc(knns_corpus c, const void* queries, int dtype, lon
   L.self_offset = p->self_offset;
   L.out64 = out64;
   L.bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
+  L.certify = (p->flags & KNNS_NO_CERTIFY) == 0;
   // the flags (overflow counter) are cleared by a fill kernel inside the sequence
   KNNS_HIP(enqueue_search(L, B, st, nullptr));
   int flags_h[16] = {0};
@@ -990,6 +1219,36 @@ knns_status do_search_noalloc(knns_corpus c, const void* queries, int dtype, lon
   KNNS_HIP(hipStreamSynchronize(st));
   if (flags_h[0] != 0)
     return fail(KNNS_ERR_OVERFLOW, "candidate pool overflow (%d events): too many exact ties", flags_h[0]);
+  if (flags_h[2] > 0 && L.certify && prec != KNNS_PREC_FP8) {
+    // uncertified queries: re-searched inside the same scratch array (the search above is done)
+    ResearchIn R;
+    R.c = c;
+    R.prec = prec;
+    R.dtype = dtype;
+    R.out64 = out64;
+    R.excl = excl;
+    R.bitonic = L.bitonic != 0;
+    R.row_begin = 0;
+    R.row_end = m;
+    R.qsrc = queries;
+    R.odist = dist;
+    R.oidx = idx;
+    std::vector<uint32_t> need(n);
+    KNNS_HIP(hipMemcpyAsync(need.data(), B.thr, n * 4, hipMemcpyDeviceToHost, st));
+    KNNS_HIP(hipStreamSynchronize(st));
+    for (long i = 0; i < n; ++i) {
+      if (!need[i]) continue;
+      R.qids.push_back((int)i);
+      R.thr.push_back(need[i]);
+      R.kq.push_back(k);
+      R.off.push_back(i * (long)k);
+      const long sr = excl ? p->self_offset + i : -1;
+      R.self.push_back((sr >= 0 && sr < m) ? (int)sr : -1);
+    }
+    ScratchWorkspace ws(base, (size_t)KNNS_NOALLOC_SCRATCH_BYTES);
+    long researched = 0;
+    return run_research(R, ws, st, cus, &researched);
+  }
   return KNNS_OK;
 }
 
@@ -1047,6 +1306,7 @@ knns_status corpus_alloc(knns_corpus* out, const void* data, int dtype, long m, 
     c->ld8 = c->dp8 + 16;
   }
   auto cleanup = [&](knns_status st) {
+    if (c->bnd) (void)hipFree(c->bnd);
     if (c->x32) (void)hipFree(c->x32);
     if (c->x16) (void)hipFree(c->x16);
     if (c->x8) (void)hipFree(c->x8);
@@ -1064,6 +1324,9 @@ knns_status corpus_alloc(knns_corpus* out, const void* data, int dtype, long m, 
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP8 rows"));
   if (dtype == KNNS_F64 && hipMalloc((void**)&c->x64, (size_t)m * d * 8) != hipSuccess)
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus FP64 master"));
+  // certificate maxima, cleared on the upload stream and raised by the pack kernels
+  if (hipMalloc((void**)&c->bnd, BND_SLOTS * sizeof(uint32_t)) != hipSuccess)
+    return cleanup(fail(KNNS_ERR_NOMEM, "corpus bounds"));
   *out = c;
   return KNNS_OK;
 }
@@ -1097,6 +1360,7 @@ knns_status knns_corpus_destroy(knns_corpus c) {
   if (!c) return KNNS_OK;
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lock(E.mu);
+  if (c->bnd) (void)hipFree(c->bnd);
   if (c->x32) (void)hipFree(c->x32);
   if (c->x16) (void)hipFree(c->x16);
   if (c->x8) (void)hipFree(c->x8);

@@ -91,6 +91,19 @@ struct FinalArgs {
   int32_t* out_idx;
   int n;
   const int* n_dev;      // optional: real query count on the device (overrides n)
+  // shortlist certificate (certify = 1): a query is certified when every corpus row outside its
+  // shortlist provably lies farther than its k-th exact distance. thr[q] holds the final strict
+  // threshold (ord of the K'-th approximate key; every row outside the shortlist has a key >= it);
+  // the kernel overwrites it with 0 for a certified query, else with the ord of the smallest key
+  // threshold whose admitted rows certify it, and counts such queries in flags[2].
+  int certify;
+  uint32_t* thr;
+  int* flags;
+  const double* qn2;     // ||q~||^2 per query (operand as rounded for the scan)
+  const double* qrho;    // ||q~ - q|| per query, rounded up
+  const uint32_t* bnd;   // corpus maxima (PackArgs::bnd); slot bslot
+  int bslot;
+  double gamma;          // relative bound of the scan's FP32 accumulation (all roundings of a key)
 };
 
 struct PackArgs {
@@ -110,7 +123,15 @@ struct PackArgs {
   uint8_t* x8;           // optional FP8 rows [rows_pad][ld8 bytes] (pack_fp8)
   long ld8;              // bytes per FP8 row: dp8 + 16
   int dp8;               // FP8 row bytes (64, 128 or 256)
+  // error-bound inputs of the shortlist certificate (all optional):
+  uint32_t* bnd;         // corpus rows: running maxima (float bits, atomicMax) of the rounding radius
+                         // ||x~ - x|| and of the norm ||x~|| per operand copy: [BND_RHO + slot],
+                         // [BND_NRM + slot], slot = the operand precision (Prec)
+  double* qn2;           // query rows: ||q~||^2 of the operand of precision qprec (one per row)
+  double* qrho;          // query rows: ||q~ - q|| of that operand, rounded up
+  int qprec;             // Prec of the search's operand (which copy qn2 / qrho describe)
 };
+constexpr int BND_RHO = 0, BND_NRM = 3, BND_SLOTS = 8;
 
 // Launchers (knns_kernels.hip). All enqueue on `stream` and return hipError_t.
 hipError_t launch_pack(const PackArgs& a, hipStream_t stream);
@@ -124,7 +145,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long n, hipStream_t stream);
 hipError_t launch_init_queries(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
                                int* carry_cnt, long n, long n_pad, int k, int shortlist,
                                long self_offset, long m, const int* n_dev, int write_k,
-                               hipStream_t stream);
+                               int write_thr, hipStream_t stream);
+// dst row i = src row idx[i] (rows of row_bytes, a multiple of 4)
+hipError_t launch_gather_rows(void* dst, const void* src, const int* idx, long n, long row_bytes,
+                              hipStream_t stream);
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t stream);
 bool scan_supported(int prec, int qb, int dc);
 int scan_stage_rows(int prec, int dp, int dc);

@@ -186,6 +186,29 @@ __device__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
   return group_kth_each<WAVE>([&](auto f) { for (int i = tid; i < N; i += nt) f(get(i)); }, N, K, lds);
 }
 
+// smallest float >= x (x >= 0, finite)
+__device__ __forceinline__ float f32_up(double x) {
+  const float f = (float)x;
+  return (double)f >= x ? f : __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) + 1u);
+}
+// sqrt(x) rounded up to float (the double sqrt is within half an ulp; the factor covers it)
+__device__ __forceinline__ float sqrt_up(double x) { return f32_up(__builtin_sqrt(x) * (1.0 + 0x1p-50)); }
+
+// Certificate inputs of a packed row: corpus rows raise the running maxima of the rounding radius
+// ||x~ - x|| and of ||x~|| (float bits of non-negative values order like unsigned integers); a
+// query row records ||q~||^2 and ||q~ - q|| of the search's operand.
+__device__ __forceinline__ void pack_bounds(const PackArgs& a, long row, int slot, double nrm2, double err2) {
+  if (a.is_query) {
+    if (a.qn2 && a.qprec == slot) {
+      a.qn2[row] = nrm2;
+      a.qrho[row] = (double)sqrt_up(err2);
+    }
+  } else if (a.bnd && row < a.rows) {
+    atomicMax(&a.bnd[BND_RHO + slot], __builtin_bit_cast(uint32_t, sqrt_up(err2)));
+    atomicMax(&a.bnd[BND_NRM + slot], __builtin_bit_cast(uint32_t, sqrt_up(nrm2)));
+  }
+}
+
 // ------------------------------------------------------------------------------------------------
 // pack: one wave per row
 // ------------------------------------------------------------------------------------------------
@@ -194,7 +217,7 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= a.rows_pad) return;
   const bool real = row < a.rows;
-  double n32 = 0.0, n16 = 0.0;
+  double n32 = 0.0, n16 = 0.0, e32 = 0.0, e16 = 0.0;
   for (int j = lane; j < a.dp; j += 64) {
     double v = 0.0;
     if (real && j < a.d) {
@@ -205,15 +228,23 @@ __global__ __launch_bounds__(256) void pack_kernel(PackArgs a) {
     const float f = (float)v;
     a.x32[row * a.ld32 + j] = f;
     n32 += (double)f * (double)f;
+    e32 += ((double)f - v) * ((double)f - v);
     if (a.x16) {
       const uint16_t b = f2bf(f);
       a.x16[row * a.ld16 + j] = b;
       const double bd = (double)bf2f(b);
       n16 += bd * bd;
+      e16 += (bd - v) * (bd - v);
     }
   }
   n32 = wave_sum_f64(n32);
   n16 = wave_sum_f64(n16);
+  e32 = wave_sum_f64(e32);
+  e16 = wave_sum_f64(e16);
+  if (lane == 0) {
+    pack_bounds(a, row, PREC_F32, n32, e32);
+    if (a.x16) pack_bounds(a, row, PREC_BF16, n16, e16);
+  }
   if (lane < 4) {
     float v = 0.f;
     if (a.is_query) {
@@ -299,15 +330,19 @@ __global__ __launch_bounds__(256) void pack_fp8_kernel(PackArgs a) {
   const int pe = fp8_scale_exp(amax);
   const double sc = __builtin_ldexp(1.0, pe), inv = __builtin_ldexp(1.0, -pe);
   uint8_t* out = a.x8 + row * a.ld8;
-  double nrm = 0.0;
+  double nrm = 0.0, err = 0.0;
   for (int j = lane; j < a.dp8; j += 64) {
-    const float f = (float)val(j);
+    const double v = val(j);
+    const float f = (float)v;
     const uint8_t b = f32_to_e4m3((float)((double)f * inv));
     out[j] = b;
     const double dq = e4m3_to_f64(b) * sc;
     nrm += dq * dq;
+    err += (dq - v) * (dq - v);
   }
   nrm = wave_sum_f64(nrm);
+  err = wave_sum_f64(err);
+  if (lane == 0) pack_bounds(a, row, PREC_FP8, nrm, err);
   if (lane < 4) {
     // aug unit, as 4 dwords: the BF16 terms (t1, t2), (t3, 0) of -||r~||^2 / 2 (corpus rows;
     // zeros for queries), then e + 127 in byte 8, then zeros
@@ -1393,6 +1428,47 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// shortlist certificate (one thread per query, after the exact rerank)
+// ------------------------------------------------------------------------------------------------
+// Every corpus row x outside the shortlist has an approximate key s~(x) >= tau (tau = the key of
+// the K'-th entry, thr[q]). The key is s~ = ||r~||^2/2 - q~.r~ accumulated in FP32 from operands
+// q~, r~ rounded for the scan, so with alpha = gamma (||q~|| max||r~|| + max||r~||^2 / 2) bounding
+// the accumulation error, ||q~ - r~||^2 = 2 s(q~, r~) + ||q~||^2 >= 2 (tau - alpha) + ||q~||^2, and
+// ||q - r|| >= ||q~ - r~|| - ||q~ - q|| - max ||r~ - r|| (triangle inequality). If that lower bound
+// exceeds D_k, the k-th exact distance of the shortlist, no row outside it can be among the k
+// nearest: the query is certified. Otherwise the kernel leaves in thr[q] the smallest key threshold
+// that certifies it -- every row with a key at or above it is provably farther than D_k -- for a
+// re-search that admits every row below it.
+__device__ __forceinline__ double next_up_f32(double x) {   // a float strictly above x
+  const float f = (float)x;
+  float g = (double)f > x ? f : __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) + (f >= 0.f ? 1u : -1u));
+  if (g == 0.f) g = 0x1p-149f;
+  return (double)g;
+}
+
+__device__ void certify_query(const FinalArgs& a, int q, int k, int Kq, double dk) {
+  const uint32_t t = a.thr[q];
+  uint32_t need = 0;
+  if (Kq >= k && t != ORD_POS_INF) {
+    const double qn2 = a.qn2[q], rq = a.qrho[q];
+    const double rmax = (double)__builtin_bit_cast(float, a.bnd[BND_RHO + a.bslot]);
+    const double nmax = (double)__builtin_bit_cast(float, a.bnd[BND_NRM + a.bslot]);
+    const double alpha = a.gamma * (__builtin_sqrt(qn2) * nmax + 0.5 * nmax * nmax) * (1.0 + 1e-9);
+    const double Dk = __builtin_sqrt(dk) * (1.0 + 1e-12);   // covers the rerank's own rounding
+    const double r = Dk + rq + rmax;
+    double tn = 0.5 * (r * r - qn2) + alpha;
+    tn += 1e-12 * (r * r + qn2 + 1.0);                         // FP64 rounding of this expression
+    const double tau = (double)ord2f(t);
+    if (!(tau > tn)) {
+      need = f2ord((float)next_up_f32(tn));
+      if (need == 0) need = 1;
+      atomicAdd(&a.flags[2], 1);
+    }
+  }
+  a.thr[q] = need;
+}
+
+// ------------------------------------------------------------------------------------------------
 // final kernel: one 256-thread block per query
 // ------------------------------------------------------------------------------------------------
 constexpr int FINAL_MAX = 2048;
@@ -1478,6 +1554,7 @@ __global__ __launch_bounds__(256) void final_block_kernel(FinalArgs a) {
   __syncthreads();
   for (int i = Kq + tid; i < P2; i += nt) { ek[i] = ~0ull; ev[i] = ~0u; }
   bitonic_sort_lds_pair(ek, ev, P2);
+  if (a.certify && tid == 0) certify_query(a, q, k, Kq, Kq >= k ? __builtin_bit_cast(double, ek[k - 1]) : 0.0);
   const long o = a.out_off[q];
   for (int t = tid; t < k; t += nt) {
     double dist;
@@ -1681,6 +1758,7 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
   wave_sync();
   for (int i = Kq + lane; i < P; i += 64) { dk[i] = ~0ull; di[i] = ~0u; }
   wave_bitonic_pair_r8(dk, di, P);   // 64 <= P <= FW_MAX
+  if (a.certify && lane == 0) certify_query(a, q, k, Kq, Kq >= k ? __builtin_bit_cast(double, dk[k - 1]) : 0.0);
   const long o = a.out_off[q];
   for (int t = lane; t < k; t += 64) {
     double dist;
@@ -1700,13 +1778,15 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
 // Per-query control state. n_dev (optional) holds the real query count on the device, overriding
 // n: a search captured in a graph for n_pad query slots then serves any count up to n_pad.
 // write_k = 0 leaves kc / kq / off to the caller (per-query k copied from the host).
+// write_thr = 0 keeps the real queries' thresholds (preset by the caller: a re-search).
 __global__ void init_queries_kernel(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
                                     int* carry_cnt, long n, long n_pad, int k, int shortlist,
-                                    long self_offset, long m, const int* n_dev, int write_k) {
+                                    long self_offset, long m, const int* n_dev, int write_k, int write_thr) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_pad) return;
   const long nr = n_dev ? (long)*n_dev : n;
-  thr[i] = i < nr ? ORD_POS_INF : ORD_NEG_INF;
+  if (i >= nr) thr[i] = ORD_NEG_INF;
+  else if (write_thr) thr[i] = ORD_POS_INF;
   if (write_k) kc[i] = shortlist;
   if (self_ids) {
     const long r = self_offset + i;
@@ -1724,11 +1804,29 @@ __global__ void init_queries_kernel(uint32_t* thr, int* kc, int* kq, long* off, 
 hipError_t launch_init_queries(uint32_t* thr, int* kc, int* kq, long* off, int* self_ids,
                                int* carry_cnt, long n, long n_pad, int k, int shortlist,
                                long self_offset, long m, const int* n_dev, int write_k,
-                               hipStream_t stream) {
+                               int write_thr, hipStream_t stream) {
   if (n_pad <= 0) return hipSuccess;
   hipLaunchKernelGGL(init_queries_kernel, dim3((unsigned)((n_pad + 255) / 256)), dim3(256), 0, stream,
                      thr, kc, kq, off, self_ids, carry_cnt, n, n_pad, k, shortlist, self_offset, m,
-                     n_dev, write_k);
+                     n_dev, write_k, write_thr);
+  return hipGetLastError();
+}
+
+// one thread per 4-byte word of the gathered rows
+__global__ void gather_rows_kernel(uint32_t* dst, const uint32_t* src, const int* idx, long n, long words) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * words) return;
+  const long i = t / words, w = t % words;
+  dst[t] = src[(long)idx[i] * words + w];
+}
+
+hipError_t launch_gather_rows(void* dst, const void* src, const int* idx, long n, long row_bytes,
+                              hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  if (row_bytes % 4) return hipErrorInvalidValue;
+  const long words = row_bytes / 4, total = n * words;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
+                     (uint32_t*)dst, (const uint32_t*)src, idx, n, words);
   return hipGetLastError();
 }
 

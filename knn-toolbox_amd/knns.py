@@ -28,7 +28,7 @@ KNNS_F32, KNNS_F64 = 0, 1
 PREC = {"fp32": 0, "bf16": 1, "fp8": 2, "fp64": 3}
 SELECT = {"auto": 0, "threshold": 1, "heap": 1, "bitonic": 2}
 KNNS_CORPUS_DEVICE_PTR, KNNS_CORPUS_BF16, KNNS_CORPUS_FP8 = 0x1, 0x2, 0x4
-KNNS_DEVICE_PTRS, KNNS_EXCLUDE_SELF, KNNS_DIST_F64, KNNS_KERNEL_TIMES = 0x1, 0x2, 0x4, 0x8
+KNNS_DEVICE_PTRS, KNNS_EXCLUDE_SELF, KNNS_DIST_F64, KNNS_KERNEL_TIMES, KNNS_NO_CERTIFY = 0x1, 0x2, 0x4, 0x8, 0x10
 
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "OVERFLOW", 6: "TIMEOUT"}
 
@@ -49,7 +49,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("total_s", ctypes.c_double), ("scan_s", ctypes.c_double), ("select_s", ctypes.c_double),
                 ("rounds", ctypes.c_long), ("slow_compactions", ctypes.c_long), ("candidates", ctypes.c_long),
                 ("groups", ctypes.c_int), ("slices_max", ctypes.c_int), ("qb", ctypes.c_int),
-                ("dc", ctypes.c_int), ("dp", ctypes.c_int)]
+                ("dc", ctypes.c_int), ("dp", ctypes.c_int), ("uncertified", ctypes.c_long),
+                ("researched", ctypes.c_long)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -250,15 +251,16 @@ class Corpus:
 
     def search(self, queries, k: int, precision: str = "fp32", select: str = "auto", shortlist: int = 0,
                exclude_self: bool = False, self_offset: int = 0, ratio: int = 0, slices: int = 0,
-               dist_f64: bool = False):
-        """Host arrays in, (dist (n,k), idx (n,k) int32) out, sorted by (distance, index)."""
+               dist_f64: bool = False, certify: bool = True):
+        """Host arrays in, (dist (n,k), idx (n,k) int32) out, sorted by (distance, index).
+        certify=False skips the shortlist certificate and its re-search (KNNS_NO_CERTIFY)."""
         q = np.ascontiguousarray(queries, dtype=self.dtype)
         n = q.shape[0]
         out64 = self.dtype == np.float64 or dist_f64
         D = np.empty((n, k), dtype=np.float64 if out64 else np.float32)
         I = np.empty((n, k), dtype=np.int32)
         p = self._params(precision, k, select, shortlist, exclude_self, self_offset, ratio, slices,
-                         KNNS_DIST_F64 if dist_f64 else 0)
+                         (KNNS_DIST_F64 if dist_f64 else 0) | (0 if certify else KNNS_NO_CERTIFY))
         st = native().knns_search(self._h, q.ctypes.data, KNNS_F64 if self.dtype == np.float64 else KNNS_F32,
                                   n, ctypes.byref(p), D.ctypes.data, I.ctypes.data)
         if st:

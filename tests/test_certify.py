@@ -1,0 +1,182 @@
+"""Shortlist certificate: exact top-k by construction for the FP32, BF16 and FP64 forms.
+
+The scan ranks corpus rows by an approximate key s~ = ||r~||^2/2 - q~.r~ (operands rounded for
+the MFMA, FP32 accumulation) and keeps a shortlist of K' rows; the final kernel reranks them
+exactly. A query is certified when every row outside the shortlist provably lies farther than
+the k-th exact distance D_k: with tau the K'-th key, alpha the accumulation bound and the rounding
+radii rho_q = ||q~ - q||, rho_r = max ||r~ - r||,
+
+    ||q - r|| >= sqrt(2 (tau - alpha) + ||q~||^2) - rho_q - rho_r  >  D_k.
+
+Uncertified queries are searched again with every row inside that bound admitted.
+
+CPU: the bound is a true lower bound on vectors whose rounding errors are aligned against the
+distance (the worst case of the triangle inequality), and leaving out either radius breaks it.
+GPU: on a near-tie shell (2,000 rows within 1e-3 of the same squared distance) the BF16 shortlist
+of k + k/2 + 16 provably misses true neighbours without the certificate, and the certified
+search equals the FP64 oracle for FP32, BF16 and FP64, through the native API, the packed
+batches, the no-allocation entry point and the drop-in.
+"""
+import numpy as np
+import pytest
+
+
+def _bf16(x):
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def _key_f32(q, r):
+    """s~ accumulated sequentially in FP32 from exact BF16 products (one of the orders the bound
+    covers): -||r~||^2/2 first, then the products."""
+    acc = np.float32(-0.5 * float((r.astype(np.float64) ** 2).sum()))
+    for a, b in zip(q.astype(np.float32), r.astype(np.float32)):
+        acc = np.float32(acc + np.float32(a * b))
+    return -float(acc)
+
+
+def test_bound_is_a_lower_bound_on_aligned_rounding():
+    """Worst case of the triangle inequality: in every coordinate q rounds up and r rounds down by
+    almost half a BF16 ulp while q - r points along the same positive direction, so
+    ||q~ - r~|| - ||q - r|| is within 0.4% of rho_q + rho_r."""
+    rng = np.random.default_rng(3)
+    d = 64
+    ulp = 2.0 ** -7                                   # BF16 ulp of values in [1, 2)
+    ok_full, broken_q, broken_r = True, True, True
+    for trial in range(20):
+        b = (1.0 + rng.integers(8, 120, d) * ulp)    # representable BF16 values in [1, 2)
+        step = rng.integers(1, 4) * ulp               # q - r along (1, ..., 1)
+        q = (b + 0.501 * ulp).astype(np.float32)      # rounds up by 0.499 ulp
+        r = (b - step + 0.499 * ulp).astype(np.float32)   # rounds down by 0.499 ulp
+        qt, rt = _bf16(q).astype(np.float64), _bf16(r).astype(np.float64)
+        q64, r64 = q.astype(np.float64), r.astype(np.float64)
+        assert np.all(qt > q64) and np.all(rt < r64)
+        true = np.sqrt(((q64 - r64) ** 2).sum())
+        rho_q = np.sqrt(((qt - q64) ** 2).sum())
+        rho_r = np.sqrt(((rt - r64) ** 2).sum())
+        s = _key_f32(qt, rt)
+        n = d + 16
+        gamma = n * 2.0 ** -23 / (1 - n * 2.0 ** -23) + 2.0 ** -26
+        nq, nr = np.sqrt((qt ** 2).sum()), np.sqrt((rt ** 2).sum())
+        alpha = gamma * (nq * nr + 0.5 * nr * nr)
+        inner = np.sqrt(max(0.0, 2 * (s - alpha) + (qt ** 2).sum()))
+        ok_full &= bool(inner - rho_q - rho_r <= true)
+        broken_q &= bool(inner - rho_r > true)       # a bound without the query's radius fails
+        broken_r &= bool(inner - rho_q > true)       # ... and one without the corpus radius
+    assert ok_full
+    assert broken_q and broken_r
+
+
+# ---------------------------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------------------------
+def _shell(seed=5, d=128, n_shell=2000, n_far=30000, n_q=6):
+    """Queries far from the origin (large BF16 rounding radii); around each, n_shell / n_q rows at
+    squared distance 1 + eps (eps uniform in [0, 1e-3]) and far rows at squared distance >= 4."""
+    g = np.random.default_rng(seed)
+    Q = (g.standard_normal((n_q, d)) * 1.0).astype(np.float32)
+    rows = []
+    per = n_shell // n_q
+    for i in range(n_q):
+        u = g.standard_normal((per, d))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        rad = np.sqrt(1.0 + g.uniform(0.0, 1e-3, per))
+        rows.append(Q[i].astype(np.float64) + rad[:, None] * u)
+    u = g.standard_normal((n_far, d))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    far = Q[g.integers(0, n_q, n_far)].astype(np.float64) + (2.0 + g.uniform(0, 3, n_far))[:, None] * u
+    R = np.concatenate(rows + [far]).astype(np.float32)
+    R = R[g.permutation(R.shape[0])]
+    return R, Q
+
+
+@pytest.fixture(scope="module")
+def shell(oracle_mod):
+    R, Q = _shell()
+    D, I = oracle_mod.exact_topk(Q, R, 100)
+    return R, Q, D, I
+
+
+def _case(name, **kw):
+    print(f"\n[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_near_tie_shell_is_certified_exact(gpu, knns_mod, shell, prec):
+    R, Q, D, I = shell
+    with knns_mod.Corpus(R, bf16=True) as c:
+        for k in (100, 10):
+            _case("shell", prec=prec, k=k)
+            Dg, Ig = c.search(Q, k, precision=prec)
+            st = knns_mod.last_stats()
+            assert np.array_equal(Ig, I[:, :k]) and np.array_equal(Dg, D[:, :k].astype(np.float32))
+            assert st["uncertified"] > 0 and st["researched"] == st["uncertified"]
+            if prec == "bf16":
+                # without the certificate the k + k/2 + 16 BF16 shortlist misses true neighbours
+                Dn, In = c.search(Q, k, precision=prec, certify=False)
+                assert knns_mod.last_stats()["researched"] == 0
+                assert not np.array_equal(In, I[:, :k])
+
+
+@pytest.mark.gpu
+def test_certified_variants_on_the_shell(gpu, knns_mod, oracle_mod, shell):
+    R, Q, D, I = shell
+    with knns_mod.Corpus(R, bf16=True) as c:
+        _case("shell_bitonic")
+        Db, Ib = c.search(Q, 100, precision="bf16", select="bitonic")
+        assert np.array_equal(Ib, I) and np.array_equal(Db, D.astype(np.float32))
+        # packed batches with per-batch k and self exclusion: query rows are corpus rows here
+        sizes = np.array([3, 2, 1])
+        rows = np.array([10, 20, 30])
+        Qr = np.concatenate([R[rows[b]:rows[b] + sizes[b]] for b in range(3)])
+        rr = np.concatenate([np.arange(rows[b], rows[b] + sizes[b]) for b in range(3)])
+        ks = np.array([100, 7, 50], dtype=np.int32)
+        _case("shell_batches")
+        Dp, Ip, off = c.search_batches(Qr, sizes, ks=ks, self_rows=rows, precision="bf16")
+        kq = np.repeat(ks, sizes)
+        Do, Io = oracle_mod.exact_topk(Qr, R, 100, self_ids=rr)
+        for i in range(Qr.shape[0]):
+            assert np.array_equal(Ip[off[i]:off[i + 1]], Io[i, :kq[i]])
+            assert np.array_equal(Dp[off[i]:off[i + 1]], Do[i, :kq[i]].astype(np.float32))
+
+
+@pytest.mark.gpu
+def test_certified_fp64_and_dropin_on_the_shell(gpu, knns_mod, oracle_mod, shell):
+    R, Q, D, I = shell
+    _case("shell_dropin")
+    vals, inds, _ = knns_mod.gpuknns_heap(Q, R, 100)
+    assert np.array_equal(inds.astype(np.int64), I.astype(np.int64))
+    assert np.array_equal(vals, D.astype(np.float32))
+    R64, Q64 = R.astype(np.float64), Q.astype(np.float64)
+    D64, I64 = oracle_mod.exact_topk(Q64, R64, 100)
+    _case("shell_fp64")
+    with knns_mod.Corpus(R64) as c:
+        Dg, Ig = c.search(Q64, 100, precision="fp64")
+    assert np.array_equal(Ig, I64) and np.array_equal(Dg, D64)
+
+
+@pytest.mark.gpu
+def test_certified_noalloc_on_the_shell(gpu, knns_mod, shell):
+    torch = pytest.importorskip("torch")
+    R, Q, D, I = shell
+    n, k = Q.shape[0], 100
+    with knns_mod.Corpus(R, bf16=True) as c:
+        _case("shell_noalloc")
+        q = torch.from_numpy(Q).to("cuda:0")
+        d = torch.empty((n, k), dtype=torch.float32, device="cuda:0")
+        i = torch.empty((n, k), dtype=torch.int32, device="cuda:0")
+        c.search_noalloc(q, n, k, d, i, precision="bf16")
+        torch.cuda.synchronize()
+    assert np.array_equal(i.cpu().numpy(), I) and np.array_equal(d.cpu().numpy(), D.astype(np.float32))
+
+
+@pytest.mark.gpu
+def test_easy_data_is_certified_in_one_pass(gpu, knns_mod, oracle_mod):
+    R, Q = oracle_mod.workload_c1()
+    with knns_mod.Corpus(R) as c:
+        _case("c1_certified")
+        Dg, Ig = c.search(Q, 16, precision="fp32")
+        st = knns_mod.last_stats()
+    assert st["uncertified"] == 0 and st["researched"] == 0
