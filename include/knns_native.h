@@ -85,6 +85,7 @@ typedef struct {
    * results are exact by construction; FP8 only counts them. */
   long uncertified;  /* queries whose first shortlist was not certified */
   long researched;   /* queries searched again */
+  double research_s; /* device time of the re-search (with KNNS_KERNEL_TIMES, else 0) */
 } knns_stats;
 
 /* fills *p with the defaults of a precision */
