@@ -603,6 +603,13 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     ma.kc = (const int*)B.kc;
     ma.n = (int)n;
     ma.bitonic = L.bitonic;
+    // expected union: the dense round's rows, else the carry plus about cmax * (new rows / rows
+    // seen) candidates
+    {
+      const long seen = R.begin - P.rounds[0].begin;
+      const double est = seen <= 0 ? (double)(R.end - R.begin) : (double)L.cmax * (double)(R.end - P.rounds[0].begin) / (double)seen;
+      ma.wide = est > 900.0 ? 1 : 0;
+    }
     ma.flags = (int*)B.flags;
     ma.self_ids = (const int*)B.self;
     ma.n_dev = L.n_dev;

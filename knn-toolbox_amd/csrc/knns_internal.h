@@ -68,6 +68,7 @@ struct MergeArgs {
   const int* kc;
   int n;                 // real queries
   int bitonic;           // 1: exact truncated sort (keeps exactly K); 0: histogram bound
+  int wide;              // threshold merge: hold up to 2048 entries per query in registers (else 1024)
   int* flags;
   const int* self_ids;   // optional: corpus row excluded for each query (all-kNN), -1 = none
   const int* n_dev;      // optional: real query count on the device (overrides n)

@@ -1187,7 +1187,6 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
 // footprint per wave is the source table and the radix histogram only, so several times more
 // waves are resident than with an LDS copy of the entries, and each entry finds its source with
 // one LDS read of a per-entry source map instead of a binary search over the source offsets.
-constexpr int MR_U = 16;
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
@@ -1300,9 +1299,12 @@ __device__ uint64_t wave_kth_regs(const uint64_t (&v)[U], int N, int K, uint32_t
   }
 }
 
+// U = register entries per lane (64 U entries per query): 16, or 32 for the rounds whose union may
+// exceed 1024 entries (large shortlists after a x4 growth step).
+template <int U>
 __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
   __shared__ int off_all[4][MERGE_MAX_SRC + 1];
-  __shared__ uint16_t map_all[4][64 * MR_U];
+  __shared__ uint16_t map_all[4][64 * U];
   __shared__ __attribute__((aligned(16))) uint32_t kth_all[4][4 * 256 + 32];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = blockIdx.x * 4 + wv;
@@ -1345,7 +1347,7 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
   uint64_t* cout = a.carry_out + (long)q * a.carry_cap;
   uint64_t E = ~0ull;
   int kept = 0;
-  if (N > 64 * MR_U) {
+  if (N > 64 * U) {
     // large union (only with unusual shortlists or ratios): radix select over global memory
     auto gget = [&](int i) -> uint64_t {
       int lo = 0, hi = nsrc - 1;
@@ -1378,37 +1380,37 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
     }
     wave_sync();
     // branch-free: every lane computes a valid address for each slot (idle slots read the first
-    // carry entry), so all MR_U loads are in flight together and one wait covers them; the key
+    // carry entry), so all U loads are in flight together and one wait covers them; the key
     // conversion follows in a second pass
-    uint64_t v[MR_U];
-    int src[MR_U], jj[MR_U];
-    // the map holds 64 MR_U slots and off[0] always exists: both reads are unconditional (slots
+    uint64_t v[U];
+    int src[U], jj[U];
+    // the map holds 64 U slots and off[0] always exists: both reads are unconditional (slots
     // past N read stale values that the selects discard), so they too are in flight together
 #pragma unroll
-    for (int u = 0; u < MR_U; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int m = (int)map[lane + 64 * u];
       src[u] = lane + 64 * u < N ? m : -1;
     }
 #pragma unroll
-    for (int u = 0; u < MR_U; ++u) jj[u] = off[src[u] < 0 ? 0 : src[u]];
+    for (int u = 0; u < U; ++u) jj[u] = off[src[u] < 0 ? 0 : src[u]];
     __builtin_amdgcn_sched_barrier(0);   // every offset read is issued before its first use
 #pragma unroll
-    for (int u = 0; u < MR_U; ++u) jj[u] = src[u] < 0 ? 0 : lane + 64 * u - jj[u];
+    for (int u = 0; u < U; ++u) jj[u] = src[u] < 0 ? 0 : lane + 64 * u - jj[u];
 #pragma unroll
-    for (int u = 0; u < MR_U; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int s = src[u] < 0 ? 0 : src[u];
       const uint64_t* p = s == 0 ? cin + jj[u] : sub + (long)(s - 1) * a.sub_cap + jj[u];
       v[u] = *p;
     }
 #pragma unroll
-    for (int u = 0; u < MR_U; ++u) {
+    for (int u = 0; u < U; ++u) {
       const uint64_t e = v[u];
       v[u] = src[u] < 0 ? ~0ull
              : (src[u] == 0 ? e : ((uint32_t)e == self ? ~0ull : pool_key(e)));
     }
-    if (N > K) E = wave_kth_regs<MR_U>(v, N, K, kth_all[wv]);
+    if (N > K) E = wave_kth_regs<U>(v, N, K, kth_all[wv]);
 #pragma unroll
-    for (int u = 0; u < MR_U; ++u) {
+    for (int u = 0; u < U; ++u) {
       if (64 * u >= N) break;   // wave-uniform
       const bool keep = lane + 64 * u < N && v[u] <= E && v[u] != ~0ull;
       const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
@@ -1924,8 +1926,10 @@ hipError_t launch_merge(const MergeArgs& a, hipStream_t stream) {
   if (1 + 2 * a.slices > MERGE_MAX_SRC) return hipErrorInvalidValue;
   if (a.bitonic)
     hipLaunchKernelGGL(merge_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+  else if (a.wide)
+    hipLaunchKernelGGL(merge_reg_kernel<32>, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   else
-    hipLaunchKernelGGL(merge_reg_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(merge_reg_kernel<16>, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
