@@ -102,7 +102,8 @@ def cpu_baseline(Q, R, k) -> dict:
 
 
 def run_serving(knns, corpus, Q, seconds: float, fractions=(0.5,), max_batch: int = 4096,
-                corpus_desc: str = "1,000,000 x 128 BF16 (resident), shortlist 2k+96, certified, exact FP32 rerank") -> dict:
+                corpus_desc: str = "1,000,000 x 128 BF16 (resident), shortlist k+80+(k<32 ? k/2 : k+16), certified, exact FP32 rerank",
+                lanes: int = 0) -> dict:
     """Requests of log-uniform size in [1, 4096] queries (drawn from Q) with k uniform in
     {1, 10, 100}. Phase 1, closed loop: 8 requests kept in flight -> saturation q/s. Phase 2,
     open loop, per fraction f: Poisson arrivals (seed 9) offering f x the saturation rate ->
@@ -116,7 +117,7 @@ def run_serving(knns, corpus, Q, seconds: float, fractions=(0.5,), max_batch: in
         s0 = int(g.integers(0, Q.shape[0] - n + 1))
         return Q[s0:s0 + n], int(g.choice((1, 10, 100)))
 
-    with knns.Server(corpus, precision="bf16", max_k=100, max_batch=max_batch) as srv:
+    with knns.Server(corpus, precision="bf16", max_k=100, max_batch=max_batch, lanes=lanes) as srv:
         for _ in range(3):                        # warm the buckets
             srv.result(srv.submit(*draw()))
         srv.stats(reset=True)
@@ -177,12 +178,18 @@ def run_serving(knns, corpus, Q, seconds: float, fractions=(0.5,), max_batch: in
             st = srv.stats(reset=True)
             sweep.append({"fraction": f, "offered_qps": f * sat_qps, "achieved_qps": nq2 / el,
                           "requests_served": st["requests"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-                          "max_ms": st["max_ms"], "mean_batch": st["mean_batch"], "batches": st["batches"]})
+                          "max_ms": st["max_ms"], "mean_batch": st["mean_batch"], "batches": st["batches"],
+                          "overlapped_batches": st["overlapped"]})
             log(f"serving at {f:.2f} x saturation: {sweep[-1]}")
     out = {"corpus": corpus_desc,
            "requests": "queries per request log-uniform in [1, 4096], k uniform in {1, 10, 100}",
+           "lanes": sat["lanes"], "batching": "continuous: the next batch is packed, uploaded and launched on a "
+           "free lane while earlier ones run; overlapped_batches counts batches whose device work began "
+           "before the previous batch's ended",
            "saturation_qps": sat_qps, "saturation_mean_batch": sat["mean_batch"],
-           "saturation_p50_ms": sat["p50_ms"], "saturation_p99_ms": sat["p99_ms"], "sweep": sweep}
+           "saturation_p50_ms": sat["p50_ms"], "saturation_p99_ms": sat["p99_ms"],
+           "saturation_batches": sat["batches"], "saturation_overlapped_batches": sat["overlapped"],
+           "researched_queries": sat["researched"], "sweep": sweep}
     # the half-load point (or the first) is the headline latency
     pick = next((x for x in sweep if abs(x["fraction"] - 0.5) < 1e-9), sweep[0])
     out.update({k: pick[k] for k in ("offered_qps", "achieved_qps", "requests_served", "p50_ms", "p99_ms",
@@ -201,7 +208,7 @@ def run_c5(args, knns, workloads, torch) -> dict:
     log(f"C5 corpus resident in {time.time() - t0:.1f}s")
     m, d = R.shape
     del R
-    desc = f"{m:,} x {d} BF16 (resident, FP32 master), shortlist 2k+96, certified, exact FP32 rerank"
+    desc = f"{m:,} x {d} BF16 (resident, FP32 master), shortlist k+80+(k<32 ? k/2 : k+16), certified, exact FP32 rerank"
     # the batch cap trades throughput for tail latency: a batch of 16384 queries on this corpus
     # runs ~90 ms, and requests that arrive meanwhile wait for it
     runs = {mb: run_serving(knns, corpus, Q, args.serving_seconds, fractions=(0.25, 0.5, 0.75, 0.9),
@@ -372,6 +379,8 @@ def main() -> int:
     ap.add_argument("--skip-fp32", action="store_true")
     ap.add_argument("--skip-host-io", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--skip-c5", action="store_true",
+                    help="skip the serving_c5 block (20M x 128 resident corpus, half-load Poisson serving)")
     ap.add_argument("--skip-variants", action="store_true",
                     help="skip the entry-point / k variants (threshold and bitonic, k in {10, 100})")
     ap.add_argument("--serving-max-batch", type=int, default=4096,
@@ -541,6 +550,21 @@ def main() -> int:
         cpu = cpu_baseline(Q, R, k)
         log(f"cpu baseline: {cpu}")
 
+    # ---- serving on the C5 corpus: 20M x 128 resident (BF16 + FP32 master), half-load Poisson ----
+    serving_c5 = None
+    if not args.skip_serving and not args.skip_c5:
+        t0 = time.time()
+        R5, Q5 = workloads.workload_c5()
+        log(f"C5 generated {R5.shape} in {time.time() - t0:.1f}s")
+        c5 = knns.Corpus(R5, bf16=True)
+        m5 = R5.shape[0]
+        del R5
+        serving_c5 = run_serving(knns, c5, Q5, args.serving_seconds, fractions=(0.5,), max_batch=4096,
+                                 corpus_desc=f"{m5:,} x 128 BF16 (resident, FP32 master), shortlist 2k+96, "
+                                             "certified, exact FP32 rerank")
+        c5.close()
+        log(f"serving C5: {serving_c5}")
+
     lat_ms = np.array(lat) * 1e3
     out = {
         "metric": "queries/s at exact top-k, BF16+rerank vs FP32; p50/p99 serving latency",
@@ -557,7 +581,7 @@ def main() -> int:
         "data": "synthetic (1024-centre Gaussian mixture, numpy PCG64 seeds 2/3)",
         "config": {"workload": "C2: corpus 1,000,000 x 128 FP32 (BF16 copy), 10,000 queries, k=100",
                    "m": m, "n": n, "d": d, "k": k,
-                   "shortlist": args.shortlist or 2 * k + 96, "parallelism": "single GPU"},
+                   "shortlist": args.shortlist or (k + 80 + (k + 16 if k >= 32 else k // 2)), "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
                      "traffic": measured_traffic(m, n, k),
@@ -572,6 +596,7 @@ def main() -> int:
         "speedup_vs_fp32": (qps / fp32["qps"]) if fp32 else None,
         "variants": variants,
         "host_boundary": host_io,
+        "serving_c5": serving_c5,
         "p50_ms": serving["p50_ms"] if serving else None,
         "p99_ms": serving["p99_ms"] if serving else None,
         "serving": serving,

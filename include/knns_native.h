@@ -68,6 +68,8 @@ typedef struct {
   int ratio;         /* corpus growth factor between threshold rounds; 0 = planner */
   int slices;        /* corpus slices per query; 0 = planner */
   void* stream;      /* hipStream_t to run on; NULL = the engine's stream */
+  int qb;            /* query blocks of 32 per scan wave (tile shape: 1, 2; 3, 4 for BF16 with
+                        d <= 128); 0 = planner / tuning table; unsupported values fall back */
 } knns_params;
 
 typedef struct {
@@ -155,7 +157,7 @@ knns_status knns_last_stats(knns_stats* out);
 int knns_device_count(void);
 /* releases the cached workspace of the engine */
 void knns_release_workspace(void);
-/* Replaces the planner's tuning table (text: "precision d k n min_waves ratio [ms]" per line,
+/* Replaces the planner's tuning table (text: "precision d k n m min_waves ratio qb [ms]" per line,
  * as written by knn-toolbox_amd/tune.py); NULL or "" clears it (built-in defaults). Without a
  * call the table is read from $KNNS_TUNING_TABLE or <library dir>/../tuning/gfx950.tsv. */
 knns_status knns_set_tuning_table(const char* path);
@@ -163,10 +165,14 @@ knns_status knns_set_tuning_table(const char* path);
 /* ---- serving: continuous batching over a resident corpus ----
  * No counterpart in the reference (its API is one synchronous call per query batch); this is
  * the serving form of the same search for callers that submit many independent requests.
- * A server owns one device workspace and one captured HIP graph per batch-capacity bucket
- * (64, 256, 1024, ... up to max_batch query slots); a worker thread packs every pending request
- * (FIFO) into the smallest bucket that holds them and replays that bucket's graph. A request's
- * result does not depend on what it is batched with: it equals knns_search on its own. */
+ * Requests are split into pieces packed, in arrival order, into batches of query slots. The
+ * server runs `lanes` batches at once (default 2), each lane with its own device workspace, stream
+ * and captured HIP graph per batch-capacity bucket (64, 256, 1024, ... up to max_batch slots): a
+ * dispatcher thread packs, uploads and launches the next batch on a free lane while the previous
+ * one runs, so their kernels overlap on the device, and a completer thread retires batches in
+ * order. Work that does not fit the free lanes waits in the queue (deferred, never refused); a
+ * request larger than a batch is served in several. A request's result does not depend on what
+ * it is batched with: it equals knns_search on its own (shortlists certified, as there). */
 typedef struct knns_server_s* knns_server;
 
 typedef struct {
@@ -174,22 +180,27 @@ typedef struct {
   int max_k;           /* largest k a request may ask for, 1..1024 */
   long max_batch;      /* query slots of the largest bucket (>= 1) */
   long mem_budget;     /* device bytes the server may hold; 0 = no limit. Buckets whose workspace
-                          does not fit are dropped at creation (the largest first) */
+                          (times the lanes) does not fit are dropped at creation (the largest first) */
+  int lanes;           /* batches in flight at once, 1..3; 0 = 2 */
 } knns_server_config;
 
 typedef struct {
   long requests, queries, batches;
   double p50_ms, p99_ms, max_ms;  /* request latency: submit to results written */
   double mean_batch;              /* queries per batch */
-  double busy_s;                  /* time the worker spent running batches */
+  double busy_s;                  /* wall time from the first launch to the last retired batch */
   long buckets, max_batch;        /* after the memory budget */
-  long workspace_bytes;
+  long workspace_bytes;           /* all lanes */
+  long lanes;
+  long overlapped;                /* batches whose device work began before the previous one ended */
+  double device_s;                /* summed device time of the batches (upload to download) */
+  long researched;                /* queries searched again by the shortlist certificate */
 } knns_server_stats;
 
 knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_server_config* cfg);
 /* Copies the n queries (host memory, dtype of the corpus); results are written to dist (n*k,
  * float, or double for double corpora) and idx (n*k int32), which must stay valid until
- * knns_server_wait returns for the ticket. n must not exceed the server's max_batch. */
+ * knns_server_wait returns for the ticket. n may exceed max_batch (the request is split). */
 knns_status knns_server_submit(knns_server s, const void* queries, int dtype, long n, int k,
                                void* dist, int32_t* idx, unsigned long long* ticket);
 /* Blocks until the request is done (timeout_s < 0: no limit) and returns its status; a ticket

@@ -53,13 +53,14 @@ struct Plan {
 struct Tuning {
   int min_waves = 5;   // slices: the smallest count filling >= min_waves waves of resident WGs
   int ratio = 0;       // corpus growth between threshold rounds; 0 = x4 below m/16, then x2
+  int qb = 0;          // query blocks of 32 per scan wave (tile shape); 0 = the built-in rule
 };
-// Nearest table entry for (precision, d, k, n) (log distance over d, k, n); defaults without a
-// table or without an entry of that precision.
-Tuning tuning_for(int prec, int d, int k, long n);
+// Nearest table entry for (precision, d, k, n, m) (log distance over d, k, n, m); defaults
+// without a table or without an entry of that precision.
+Tuning tuning_for(int prec, int d, int k, long n, long m);
 
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
-               int slices_req, int min_waves = 5, bool allow_dense = true);
+               int slices_req, int min_waves = 5, bool allow_dense = true, int qb_req = 0);
 
 // Relative error bound of the scan's FP32 accumulation of one key (every rounding of it), for
 // the certificate: gamma = n u / (1 - n u), n = roundings per key, u = 2^-23 (covers rounding
@@ -148,6 +149,8 @@ struct ResearchIn {
 // writes their k results at their offsets and synchronises `st`. Fails when a re-search still
 // cannot be certified.
 knns_status run_research(const ResearchIn& R, Workspace& ws, hipStream_t st, int cus, long* researched);
+// run_research on the engine's own grow-only workspace (serialised with every engine call)
+knns_status research_on_engine(const ResearchIn& R, hipStream_t st, long* researched);
 
 // Timing hook: called with 0/1 around each scan launch, 2 after each merge, 3/4 around the final
 // kernel (used to record events; nullptr records nothing).

@@ -55,7 +55,9 @@ struct Engine {
   int device = 0;
   int cus = 256;
   hipStream_t stream = nullptr;
-  hipStream_t ustream = nullptr;   // corpus uploads that overlap a search (knns_search_once)
+  hipStream_t ustream = nullptr;   // corpus uploads that overlap a search (knns_search_once); created
+                                   // at its first use: a process has few hardware queues (4 by
+                                   // default) and streams beyond them share queues and serialise
   void* buf[W_NSLOTS] = {};
   size_t cap[W_NSLOTS] = {};
   std::vector<hipEvent_t> events;
@@ -71,7 +73,6 @@ struct Engine {
     KNNS_HIP(hipGetDeviceProperties(&prop, device));
     cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     KNNS_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    KNNS_HIP(hipStreamCreateWithFlags(&ustream, hipStreamNonBlocking));
     ready = true;
     return KNNS_OK;
   }
@@ -159,13 +160,13 @@ Engine& engine() {
 }  // namespace
 
 // ---- tuning table ----
-// Text file, one entry per line: precision d k n min_waves ratio [measured ms]; '#' starts a
-// comment. Loaded at the first search from $KNNS_TUNING_TABLE, else from
+// Text file, one entry per line: precision d k n m min_waves ratio qb [measured ms] (m = corpus
+// rows, qb = query blocks per scan wave, 0 = built-in rule); '#' starts a comment. Loaded at the first search from $KNNS_TUNING_TABLE, else from
 // <directory of libknns_amd.so>/../tuning/gfx950.tsv; knns_set_tuning_table replaces it.
 namespace {
 struct TuneEntry {
   int prec, d, k;
-  long n;
+  long n, m;
   Tuning t;
 };
 std::mutex g_tune_mu;
@@ -187,9 +188,10 @@ knns_status load_tuning(const char* path, std::vector<TuneEntry>* out) {
     if (*p == '\0' || *p == '\n' || *p == '\r') continue;
     char prec[16];
     TuneEntry e{};
-    if (std::sscanf(p, "%15s %d %d %ld %d %d", prec, &e.d, &e.k, &e.n, &e.t.min_waves, &e.t.ratio) != 6) {
+    if (std::sscanf(p, "%15s %d %d %ld %ld %d %d %d", prec, &e.d, &e.k, &e.n, &e.m, &e.t.min_waves, &e.t.ratio,
+                    &e.t.qb) != 8) {
       std::fclose(f);
-      return fail(KNNS_ERR_INVALID, "%s:%d: expected 'precision d k n min_waves ratio'", path, ln);
+      return fail(KNNS_ERR_INVALID, "%s:%d: expected 'precision d k n m min_waves ratio qb'", path, ln);
     }
     if (!std::strcmp(prec, "fp32")) e.prec = KNNS_PREC_FP32;
     else if (!std::strcmp(prec, "bf16")) e.prec = KNNS_PREC_BF16;
@@ -199,8 +201,8 @@ knns_status load_tuning(const char* path, std::vector<TuneEntry>* out) {
       std::fclose(f);
       return fail(KNNS_ERR_INVALID, "%s:%d: unknown precision '%s'", path, ln, prec);
     }
-    if (e.d < 1 || e.k < 1 || e.n < 1 || e.t.min_waves < 1 || e.t.min_waves > 64 ||
-        (e.t.ratio != 0 && (e.t.ratio < 2 || e.t.ratio > 16))) {
+    if (e.d < 1 || e.k < 1 || e.n < 1 || e.m < 1 || e.t.min_waves < 1 || e.t.min_waves > 64 ||
+        (e.t.ratio != 0 && (e.t.ratio < 2 || e.t.ratio > 16)) || e.t.qb < 0 || e.t.qb > 4) {
       std::fclose(f);
       return fail(KNNS_ERR_INVALID, "%s:%d: value out of range", path, ln);
     }
@@ -224,7 +226,7 @@ std::string default_tuning_path() {
 }
 }  // namespace
 
-Tuning tuning_for(int prec, int d, int k, long n) {
+Tuning tuning_for(int prec, int d, int k, long n, long m) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (!g_tune_loaded) {
     g_tune_loaded = true;
@@ -239,7 +241,7 @@ Tuning tuning_for(int prec, int d, int k, long n) {
   for (const TuneEntry& e : g_tune) {
     if (e.prec != prec) continue;
     const double dist = std::fabs(std::log2((double)d / e.d)) + std::fabs(std::log2((double)k / e.k)) +
-                        std::fabs(std::log2((double)n / e.n));
+                        std::fabs(std::log2((double)n / e.n)) + std::fabs(std::log2((double)m / e.m));
     if (dist < bd) { bd = dist; best = e.t; }
   }
   return best;
@@ -262,36 +264,38 @@ void dims_for(int d, int* dp, int* dc) {
 // Shortlist sizes K' per precision. Exactness does not depend on them (the certificate re-searches
 // any query whose shortlist it cannot prove complete); they set how often that happens. The
 // certificate needs the gap between the k-th exact distance and the K'-th approximate key to exceed
-// the key error bound, which grows with the operand rounding: measured on a 1M x 128 mixture
-// corpus with 10k queries, BF16 at 2k + 96 re-searches 16 queries at k = 100 and none at k = 1, 10;
-// FP32 at k + k/4 + 16 none at k = 10, 100 (k + 16 re-searched 1,465 at k = 100).
+// the key error bound, which grows with the operand rounding. Measured on a 1M x 128 mixture
+// corpus with 10k queries: BF16 at k = 100 re-searches 861 queries with K' = 266, 16 with 296 and 1
+// with 316; at k = 10, 333 with 61, 15 with 76, 1 with 90; at k = 1, 3 with 33 and none with 66.
+// FP32 at k + k/4 + 16 re-searches none at k = 10, 100 (k + 16 re-searched 1,465 at k = 100).
 int default_shortlist(int prec, int k) {
   switch (prec) {
-    case KNNS_PREC_BF16: return 2 * k + 96;
+    case KNNS_PREC_BF16: return k + 80 + (k >= 32 ? k + 16 : k / 2);
     case KNNS_PREC_FP8: return 4 * k + 32;
     default: return k + k / 4 + 16;
   }
 }
 
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
-               int slices_req, int min_waves, bool allow_dense) {
+               int slices_req, int min_waves, bool allow_dense, int qb_req) {
   Plan P{};
   dims_for(d, &P.dp, &P.dc);
   P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : (prec == KNNS_PREC_FP8 ? PREC_FP8 : PREC_F32);
   if (P.prec_kernel == PREC_FP8) P.dc = fp8_row_bytes(d);
   P.qb = (P.prec_kernel != PREC_F32 && n > 32) ? 2 : 1;
+  if (qb_req > 0 && scan_qb_ok(P.prec_kernel, P.dp, P.dc, qb_req)) P.qb = qb_req;
   const int qw = 32 * P.qb;
   P.groups = (int)round_up((n + qw - 1) / qw, 4);  // a scan workgroup runs 4 query groups
   P.n_pad = P.groups * qw;
   P.kmax = kmax;
   P.cmax = cmax;
-  // scan workgroups: 4 query groups each, 2 resident per CU. Choose the slice count so that the
+  // scan workgroups: 4 query groups each, 2 (or 1 for QB >= 3) resident per CU. Choose the slice count so that the
   // workgroups of a large round fill whole waves of resident workgroups (a last wave that is
   // mostly empty leaves the chip idle): the smallest S with >= min_waves waves and >= 95% of the
   // last wave used, else the S with the best fill. min_waves comes from the tuning table
   // (default 5: on 1M x 128 with 10k queries 5 waves beat 4 by 5%, more add merge sources).
   const long wg_rows = P.groups / 4;
-  const long cap = (long)cus * 2;
+  const long cap = (long)cus * scan_wgs_per_cu(P.qb);
   int smax = 1;
   {
     double best = -1.0;
@@ -680,7 +684,7 @@ knns_status run_research(const ResearchIn& R, Workspace& ws, hipStream_t st, int
   const size_t row_bytes = (size_t)c->d * qsz;
   // plan: the preset thresholds replace the dense first round; the slice count is lowered until
   // the workspace fits the provider's budget (the answer does not depend on it)
-  const Tuning tu = tuning_for(R.prec, c->d, kmax, nf);
+  const Tuning tu = tuning_for(R.prec, c->d, kmax, nf, m);
   int sreq = 0;
   Plan P;
   SearchSizes z;
@@ -688,7 +692,7 @@ knns_status run_research(const ResearchIn& R, Workspace& ws, hipStream_t st, int
   long n_pad = 0;
   for (;;) {
     // few rounds: the thresholds start where the certificate needs them and rarely tighten
-    P = make_plan(m, c->d, nf, kmax, cap, R.prec, cus, 64, sreq, tu.min_waves, false);
+    P = make_plan(m, c->d, nf, kmax, cap, R.prec, cus, 64, sreq, tu.min_waves, false, tu.qb);
     z = search_sizes(c, P, nf, 0, R.excl, R.out64);
     n_pad = P.n_pad;
     ctl_bytes = (size_t)nf * (4 + 4 + 8 + 4) + (size_t)n_pad * (4 + 4) + 64;
@@ -797,6 +801,18 @@ struct EngineWorkspace : Workspace {
   }
 };
 
+}  // namespace
+
+knns_status research_on_engine(const ResearchIn& R, hipStream_t st, long* researched) {
+  Engine& E = engine();
+  std::lock_guard<std::recursive_mutex> lock(E.mu);
+  if (knns_status s = E.init()) return s;
+  EngineWorkspace ws(E);
+  return run_research(R, ws, st, E.cus, researched);
+}
+
+namespace {
+
 // Searches corpus rows [row_begin, row_end) (row_end < 0: to the last row); indices are corpus rows.
 // self_host (optional, n entries): corpus row query i never returns (< 0: none); it replaces
 // KNNS_EXCLUDE_SELF's self_offset + i.
@@ -869,9 +885,9 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     }
   }
 
-  const Tuning tu = tuning_for(prec, c->d, kmax, n);
+  const Tuning tu = tuning_for(prec, c->d, kmax, n, m);
   Plan P = make_plan(m, c->d, n, kmax, cmax, prec, E.cus, p->ratio > 0 ? p->ratio : tu.ratio, p->slices,
-                     tu.min_waves);
+                     tu.min_waves, true, p->qb > 0 ? p->qb : tu.qb);
   for (Round& r : P.rounds) {   // planned over the range's rows, from its first row
     r.begin += row_begin;
     r.end += row_begin;
@@ -1178,12 +1194,13 @@ knns_status do_search_noalloc(knns_corpus c, const void* queries, int dtype, lon
   cmax = std::max(std::min((int)std::min<long>(cmax, m_eff), 2048), 1);
   // the planner's slice count, lowered until the workspace fits the scratch array (the answer does
   // not depend on the slice count, only the parallelism of the scan does)
-  const Tuning tu = tuning_for(prec, c->d, k, n);
+  const Tuning tu = tuning_for(prec, c->d, k, n, m);
   int sreq = p->slices;
   Plan P;
   SearchSizes z;
   for (;;) {
-    P = make_plan(m, c->d, n, k, cmax, prec, cus, p->ratio > 0 ? p->ratio : tu.ratio, sreq, tu.min_waves);
+    P = make_plan(m, c->d, n, k, cmax, prec, cus, p->ratio > 0 ? p->ratio : tu.ratio, sreq, tu.min_waves, true,
+                  p->qb > 0 ? p->qb : tu.qb);
     z = search_sizes(c, P, n, n * (long)k, excl, out64);
     if (carve_bytes(z) <= (size_t)KNNS_NOALLOC_SCRATCH_BYTES) break;
     if (P.slices_max <= 1)
@@ -1456,6 +1473,12 @@ knns_status knns_search_once(const void* data, int dtype, long m, int d, const v
   if (knns_status st = corpus_alloc(&c, data, dtype, m, d, 0)) return st;
   // the corpus goes up on the upload stream; each scan round waits only for the rows it reads
   Upload u;
+  if (!E.ustream && hipStreamCreateWithFlags(&E.ustream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    E.ustream = nullptr;
+    knns_corpus_destroy(c);
+    return fail(KNNS_ERR_DEVICE, "upload stream creation failed");
+  }
   knns_status st = u.start(c, data, false, E.ustream);
   if (st == KNNS_OK) st = u.next();
   if (st == KNNS_OK) st = do_search(c, queries, dtype, n, nullptr, p->k, p, dist, idx, 0, -1, &u);
@@ -1469,8 +1492,8 @@ knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns
   if (!out || m < 1 || d < 1 || n < 1 || k < 1 || k > 1024)
     return fail(KNNS_ERR_INVALID, "bad plan arguments");
   const int cl = std::min<long>(std::min(default_shortlist(precision, k), 2048), m);
-  const Tuning tu = tuning_for(precision, d, k, n);
-  Plan P = make_plan(m, d, n, k, cl, precision, 256, tu.ratio, 0, tu.min_waves);
+  const Tuning tu = tuning_for(precision, d, k, n, m);
+  Plan P = make_plan(m, d, n, k, cl, precision, 256, tu.ratio, 0, tu.min_waves, true, tu.qb);
   std::memset(out, 0, sizeof(*out));
   out->rounds = (long)P.rounds.size();
   out->groups = P.groups;

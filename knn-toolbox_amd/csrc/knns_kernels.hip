@@ -204,8 +204,13 @@ __device__ __forceinline__ void pack_bounds(const PackArgs& a, long row, int slo
       a.qrho[row] = (double)sqrt_up(err2);
     }
   } else if (a.bnd && row < a.rows) {
-    atomicMax(&a.bnd[BND_RHO + slot], __builtin_bit_cast(uint32_t, sqrt_up(err2)));
-    atomicMax(&a.bnd[BND_NRM + slot], __builtin_bit_cast(uint32_t, sqrt_up(nrm2)));
+    // an atomic only when the row raises the running maximum (the maxima settle after a few rows;
+    // a stale read only costs an unneeded atomic)
+    const uint32_t rb = __builtin_bit_cast(uint32_t, sqrt_up(err2)), nb = __builtin_bit_cast(uint32_t, sqrt_up(nrm2));
+    if (rb > __hip_atomic_load(&a.bnd[BND_RHO + slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&a.bnd[BND_RHO + slot], rb);
+    if (nb > __hip_atomic_load(&a.bnd[BND_NRM + slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&a.bnd[BND_NRM + slot], nb);
   }
 }
 
@@ -794,7 +799,7 @@ struct SwzGeom {
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 template <int QB, int SU, bool F8>
-__global__ __launch_bounds__(256, 2) void scan_swz_kernel(ScanArgs a) {
+__global__ __launch_bounds__(256, QB >= 3 ? 1 : 2) void scan_swz_kernel(ScanArgs a) {
   using G = SwzGeom<SU>;
   constexpr int ESZ = F8 ? 1 : 2;                // bytes per element
   constexpr int J = F8 ? SU / 4 : SU / 2;        // MFMA k-steps per row block
@@ -1879,6 +1884,15 @@ bool scan_supported(int prec, int qb, int dc) {
   return dc == 32 || dc == 64 || dc == 96 || dc == 128;
 }
 
+// Query blocks of 32 per wave that the scan is built for: 1 and 2 everywhere; 3 and 4 for the
+// single-chunk BF16 scan of 64- and 128-dim rows (one resident workgroup per CU: their queries
+// and accumulators take 400+ registers per lane).
+bool scan_qb_ok(int prec, int dp, int dc, int qb) {
+  if (qb == 1 || qb == 2) return true;
+  return (qb == 3 || qb == 4) && prec == PREC_BF16 && dp == dc && (dc == 64 || dc == 128);
+}
+int scan_wgs_per_cu(int qb) { return qb >= 3 ? 1 : 2; }
+
 // Row granularity of the stages (64 or 128 rows): slice sizes are multiples of it. A slice may
 // start at any row; the corpus carries 128 rows past round_up(m, 128), so the last stage of a slice
 // (whose rows past row_end are masked) never reads past the allocation. Every row stride is a
@@ -1906,6 +1920,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
     if (qb == B && dc == D) return scan_grid(scan_swz_kernel<B, D / 8, false>, a, groups, stream);
     KNNS_SWZ_CASE(1, 32) KNNS_SWZ_CASE(1, 64) KNNS_SWZ_CASE(1, 128)
     KNNS_SWZ_CASE(2, 32) KNNS_SWZ_CASE(2, 64) KNNS_SWZ_CASE(2, 128)
+    KNNS_SWZ_CASE(3, 64) KNNS_SWZ_CASE(3, 128) KNNS_SWZ_CASE(4, 64) KNNS_SWZ_CASE(4, 128)
 #undef KNNS_SWZ_CASE
     return hipErrorInvalidValue;
   }

@@ -42,7 +42,8 @@ class KnnsError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("select", ctypes.c_int), ("k", ctypes.c_int),
                 ("shortlist", ctypes.c_int), ("flags", ctypes.c_uint), ("self_offset", ctypes.c_long),
-                ("ratio", ctypes.c_int), ("slices", ctypes.c_int), ("stream", ctypes.c_void_p)]
+                ("ratio", ctypes.c_int), ("slices", ctypes.c_int), ("stream", ctypes.c_void_p),
+                ("qb", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -62,14 +63,15 @@ class Times(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("max_k", ctypes.c_int), ("max_batch", ctypes.c_long),
-                ("mem_budget", ctypes.c_long)]
+                ("mem_budget", ctypes.c_long), ("lanes", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_long), ("queries", ctypes.c_long), ("batches", ctypes.c_long),
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
                 ("mean_batch", ctypes.c_double), ("busy_s", ctypes.c_double), ("buckets", ctypes.c_long),
-                ("max_batch", ctypes.c_long), ("workspace_bytes", ctypes.c_long)]
+                ("max_batch", ctypes.c_long), ("workspace_bytes", ctypes.c_long), ("lanes", ctypes.c_long),
+                ("overlapped", ctypes.c_long), ("device_s", ctypes.c_double), ("researched", ctypes.c_long)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -238,7 +240,7 @@ class Corpus:
     def __exit__(self, *a):
         self.close()
 
-    def _params(self, precision, k, select, shortlist, exclude_self, self_offset, ratio, slices, flags):
+    def _params(self, precision, k, select, shortlist, exclude_self, self_offset, ratio, slices, flags, qb=0):
         p = Params()
         native().knns_params_init(ctypes.byref(p), PREC[precision], int(k))
         p.select = SELECT[select]
@@ -247,6 +249,7 @@ class Corpus:
         p.self_offset = int(self_offset)
         p.ratio = int(ratio)
         p.slices = int(slices)
+        p.qb = int(qb)
         return p
 
     def search(self, queries, k: int, precision: str = "fp32", select: str = "auto", shortlist: int = 0,
@@ -286,11 +289,11 @@ class Corpus:
 
     def search_device(self, q_ptr, n: int, k: int, dist_ptr, idx_ptr, precision: str = "fp32",
                       select: str = "auto", shortlist: int = 0, ratio: int = 0, slices: int = 0,
-                      stream: int = 0, kernel_times: bool = False):
+                      stream: int = 0, kernel_times: bool = False, qb: int = 0):
         """Device pointers (ints or torch tensors); synchronous. kernel_times: fill scan_s and
-        select_s of last_stats() (one event per launch)."""
+        select_s of last_stats() (one event per launch). qb: query blocks per scan wave."""
         p = self._params(precision, k, select, shortlist, False, 0, ratio, slices,
-                         KNNS_DEVICE_PTRS | (KNNS_KERNEL_TIMES if kernel_times else 0))
+                         KNNS_DEVICE_PTRS | (KNNS_KERNEL_TIMES if kernel_times else 0), qb)
         p.stream = stream or None
         st = native().knns_search(self._h, _ptr(q_ptr) if not isinstance(q_ptr, int) else q_ptr,
                                   KNNS_F64 if self.dtype == np.float64 else KNNS_F32, n, ctypes.byref(p),
@@ -450,10 +453,10 @@ class Server:
     ticket at once, result() blocks until that request's (dist, idx) are ready."""
 
     def __init__(self, corpus: "Corpus", precision: str = "bf16", max_k: int = 100, max_batch: int = 4096,
-                 mem_budget: int = 0):
+                 mem_budget: int = 0, lanes: int = 0):
         self._corpus = corpus           # keeps the corpus alive
         self._dtype = corpus.dtype
-        cfg = ServerConfig(PREC[precision], int(max_k), int(max_batch), int(mem_budget))
+        cfg = ServerConfig(PREC[precision], int(max_k), int(max_batch), int(mem_budget), int(lanes))
         h = ctypes.c_void_p()
         st = native().knns_server_create(ctypes.byref(h), corpus._h, ctypes.byref(cfg))
         if st:

@@ -78,7 +78,9 @@ def test_native_params_layout(knns_mod):
     p = knns_mod.Params()
     knns_mod.native().knns_params_init(ctypes.byref(p), 1, 7)
     assert (p.precision, p.k, p.select, p.shortlist, p.flags) == (1, 7, 0, 0, 0)
-    assert ctypes.sizeof(knns_mod.Params) == 48
+    # layout of knns_params / knns_stats as gcc lays them out from include/knns_native.h
+    assert ctypes.sizeof(knns_mod.Params) == 56 and knns_mod.Params.qb.offset == 48
+    assert ctypes.sizeof(knns_mod.Stats) == 96 and knns_mod.Stats.research_s.offset == 88
 
 
 def test_planner_shapes(knns_mod):

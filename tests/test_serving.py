@@ -1,10 +1,14 @@
-"""Serving loop (knns_server_*): continuous batching over a resident corpus with one captured HIP
-graph per batch-capacity bucket.
+"""Serving loop (knns_server_*): continuous batching over a resident corpus. Several lanes (each a
+workspace, a stream and one captured HIP graph per batch-capacity bucket) run batches at once, so a
+batch is packed, uploaded and launched while the previous one runs; requests larger than a batch
+are split across batches, work beyond the free lanes waits in the queue.
 
 Contract: a request's results are identical whether it is served alone or batched with others,
 and identical to an eager knns_search of its queries (same indices, same distances). The GPU
 tests replay every bucket graph many times with eager searches in between and compare every
-replay. CPU tests cover argument checks that run before any device work and the struct layouts."""
+replay, show device overlap between consecutive batches (their device intervals, from events),
+and serve requests larger than the largest bucket. CPU tests cover argument checks that run before
+any device work and the struct layouts."""
 import ctypes
 
 import numpy as np
@@ -12,19 +16,21 @@ import pytest
 
 
 def test_server_struct_layouts(knns_mod):
-    assert ctypes.sizeof(knns_mod.ServerConfig) == 24
-    assert ctypes.sizeof(knns_mod.ServerStats) == 88
+    # as gcc lays out include/knns_native.h
+    assert ctypes.sizeof(knns_mod.ServerConfig) == 32 and knns_mod.ServerConfig.lanes.offset == 24
+    assert ctypes.sizeof(knns_mod.ServerStats) == 120 and knns_mod.ServerStats.researched.offset == 112
 
 
 def test_server_config_rejected_before_device_work(knns_mod):
     L = knns_mod.native()
     h = ctypes.c_void_p()
     fake = ctypes.c_void_p(1)   # never dereferenced: the configuration is checked first
-    for prec, max_k, max_batch, budget, want in [
-            (0, 0, 64, 0, knns_mod.KNNS_ERR_INVALID), (0, 1025, 64, 0, knns_mod.KNNS_ERR_INVALID),
-            (0, 10, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 10, 64, -1, knns_mod.KNNS_ERR_INVALID),
-            (7, 10, 64, 0, knns_mod.KNNS_ERR_INVALID)]:
-        cfg = knns_mod.ServerConfig(prec, max_k, max_batch, budget)
+    for prec, max_k, max_batch, budget, lanes, want in [
+            (0, 0, 64, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 1025, 64, 0, 0, knns_mod.KNNS_ERR_INVALID),
+            (0, 10, 0, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 10, 64, -1, 0, knns_mod.KNNS_ERR_INVALID),
+            (7, 10, 64, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 10, 64, 0, 4, knns_mod.KNNS_ERR_INVALID),
+            (0, 10, 64, 0, -1, knns_mod.KNNS_ERR_INVALID)]:
+        cfg = knns_mod.ServerConfig(prec, max_k, max_batch, budget, lanes)
         assert L.knns_server_create(ctypes.byref(h), fake, ctypes.byref(cfg)) == want
         assert not h.value
     cfg = knns_mod.ServerConfig(0, 10, 64, 0)
@@ -113,17 +119,67 @@ def test_serving_admission(gpu, knns_mod, mixture_corpus):
         with knns_mod.Server(corpus, precision="bf16", max_k=100, max_batch=4096) as big:
             full = big.stats()
         assert full["buckets"] == 4 and full["max_batch"] == 4096   # 64, 256, 1024, 4096
-        # a budget that holds the smaller buckets only: capacity shrinks, larger requests are refused
+        # a budget that holds the smaller buckets only: capacity shrinks, and a request larger than
+        # the largest remaining bucket is deferred and served in several batches, not refused
         budget = full["workspace_bytes"] // 3
         with knns_mod.Server(corpus, precision="bf16", max_k=100, max_batch=4096, mem_budget=budget) as srv:
             st = srv.stats()
             assert 1 <= st["buckets"] < 4 and st["workspace_bytes"] <= budget
             with pytest.raises(knns_mod.KnnsError):
-                srv.submit(Q[:st["max_batch"] + 1], 10)
-            with pytest.raises(knns_mod.KnnsError):
                 srv.submit(Q[:4], 101)   # k above max_k
-            D, I = srv.result(srv.submit(Q[:st["max_batch"]], 10))
-            De, Ie = corpus.search(Q[:st["max_batch"]], 10, precision="bf16")
+            big = Q[:2 * st["max_batch"] + 5]
+            D, I = srv.result(srv.submit(big, 10))
+            De, Ie = corpus.search(big, 10, precision="bf16")
             assert np.array_equal(I, Ie) and np.array_equal(D, De)
+            assert srv.stats()["batches"] >= 3
         with pytest.raises(knns_mod.KnnsError):
             knns_mod.Server(corpus, precision="bf16", max_k=100, max_batch=4096, mem_budget=1000)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [2, 1])
+def test_batches_overlap_on_the_device(gpu, knns_mod, oracle_mod, lanes):
+    """Requests submitted while earlier batches run: with two lanes the next batch is launched
+    before the previous one finished (its device interval starts before the previous ends); with
+    one lane batches run one after another. Results are the eager search's either way."""
+    R = oracle_mod.mixture(400000, 128, 256, 0.1, center_seed=51, point_seed=52)
+    Q = oracle_mod.mixture(6000, 128, 256, 0.1, center_seed=51, point_seed=53)
+    reqs = [(Q[i * 500:(i + 1) * 500], (10, 100, 1)[i % 3]) for i in range(12)]
+    print(f"\n[case] serving overlap lanes={lanes}", flush=True)
+    with knns_mod.Corpus(R, bf16=True) as corpus:
+        eager = [corpus.search(q, k, precision="bf16") for q, k in reqs]
+        with knns_mod.Server(corpus, precision="bf16", max_k=100, max_batch=1024, lanes=lanes) as srv:
+            srv.result(srv.submit(Q[:64], 10))      # warm
+            srv.stats(reset=True)
+            tickets = []
+            for q, k in reqs:                        # arrivals spread over the first batches
+                tickets.append(srv.submit(q, k))
+            got = [srv.result(t) for t in tickets]
+            st = srv.stats()
+    assert st["lanes"] == lanes and st["batches"] >= 6
+    if lanes == 2:
+        assert st["overlapped"] >= 1
+    else:
+        assert st["overlapped"] == 0
+    for (De, Ie), (D, I) in zip(eager, got):
+        assert np.array_equal(I, Ie) and np.array_equal(D, De)
+
+
+@pytest.mark.gpu
+def test_result_timeout_keeps_the_request(gpu, knns_mod, mixture_corpus):
+    """A wait that times out leaves the request live with its output buffers: waiting again
+    returns the full result."""
+    R, Q = mixture_corpus
+    print("\n[case] serving timeout", flush=True)
+    with knns_mod.Corpus(R, bf16=True) as corpus:
+        De, Ie = corpus.search(Q, 100, precision="bf16")
+        with knns_mod.Server(corpus, precision="bf16", max_k=100, max_batch=1024) as srv:
+            t = srv.submit(Q, 100)
+            try:
+                srv.result(t, timeout=0.0)
+            except TimeoutError:
+                pass
+            D, I = srv.result(t)
+            with pytest.raises(knns_mod.KnnsError):
+                srv.result(t)                        # a done ticket is gone
+    assert np.array_equal(I, Ie) and np.array_equal(D, De)
