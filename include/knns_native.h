@@ -88,6 +88,7 @@ typedef struct {
   long uncertified;  /* queries whose first shortlist was not certified */
   long researched;   /* queries searched again */
   double research_s; /* device time of the re-search (with KNNS_KERNEL_TIMES, else 0) */
+  long shortlist;    /* K' of the search (largest over its queries) */
 } knns_stats;
 
 /* fills *p with the defaults of a precision */
@@ -157,7 +158,7 @@ knns_status knns_last_stats(knns_stats* out);
 int knns_device_count(void);
 /* releases the cached workspace of the engine */
 void knns_release_workspace(void);
-/* Replaces the planner's tuning table (text: "precision d k n m min_waves ratio qb [ms]" per line,
+/* Replaces the planner's tuning table (text: "precision d k n m min_waves ratio qb sl_extra [ms]" per line,
  * as written by knn-toolbox_amd/tune.py); NULL or "" clears it (built-in defaults). Without a
  * call the table is read from $KNNS_TUNING_TABLE or <library dir>/../tuning/gfx950.tsv. */
 knns_status knns_set_tuning_table(const char* path);

@@ -36,6 +36,9 @@ inline long round_up(long x, long a) { return (x + a - 1) / a * a; }
 void dims_for(int d, int* dp, int* dc);
 int fp8_row_bytes(int d);
 int default_shortlist(int prec, int k);
+// K' of a query: the requested shortlist (at least k), else the built-in rule plus the tuning
+// offset (at least k); capped by the searched rows and 2048
+int shortlist_for(int prec, int k, int requested, int sl_extra, long m_eff);
 
 struct Round {
   long begin, end, slice_rows;
@@ -46,6 +49,7 @@ struct Round {
 // dp: padded FP32 dims of the corpus; dc: the scan's chunk (FP8: the FP8 row bytes)
 struct Plan {
   int prec_kernel, qb, dc, dp, groups, n_pad, slices_max, sub_cap, kmax, cmax;
+  int carry_cap = 0;   // entries the carry holds per query (cmax, or more with the certificate band)
   std::vector<Round> rounds;
 };
 
@@ -54,13 +58,15 @@ struct Tuning {
   int min_waves = 5;   // slices: the smallest count filling >= min_waves waves of resident WGs
   int ratio = 0;       // corpus growth between threshold rounds; 0 = x4 below m/16, then x2
   int qb = 0;          // query blocks of 32 per scan wave (tile shape); 0 = the built-in rule
+  int sl_extra = 0;    // shortlist offset from the precision's built-in rule (default_shortlist)
 };
 // Nearest table entry for (precision, d, k, n, m) (log distance over d, k, n, m); defaults
 // without a table or without an entry of that precision.
 Tuning tuning_for(int prec, int d, int k, long n, long m);
 
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
-               int slices_req, int min_waves = 5, bool allow_dense = true, int qb_req = 0);
+               int slices_req, int min_waves = 5, bool allow_dense = true, int qb_req = 0,
+               bool band = false);
 
 // Relative error bound of the scan's FP32 accumulation of one key (every rounding of it), for
 // the certificate: gamma = n u / (1 - n u), n = roundings per key, u = 2^-23 (covers rounding
@@ -114,6 +120,7 @@ struct SearchLaunch {
   bool self_preset = false;    // B.self already holds per-query self rows (the init kernel keeps them)
   bool thr_preset = false;     // B.thr already holds the real queries' initial thresholds
   bool certify = true;         // the final kernel checks each query's shortlist certificate
+  bool band = false;           // thresholds carry the certificate band (needs certify, not FP8)
   const int* n_dev = nullptr;  // optional device query count <= n (graph-captured searches)
   Upload* feed = nullptr;      // optional: corpus rows still uploading; each scan waits for its rows
 };

@@ -160,8 +160,9 @@ Engine& engine() {
 }  // namespace
 
 // ---- tuning table ----
-// Text file, one entry per line: precision d k n m min_waves ratio qb [measured ms] (m = corpus
-// rows, qb = query blocks per scan wave, 0 = built-in rule); '#' starts a comment. Loaded at the first search from $KNNS_TUNING_TABLE, else from
+// Text file, one entry per line: precision d k n m min_waves ratio qb sl_extra [measured ms]
+// (m = corpus rows, qb = query blocks per scan wave, 0 = built-in rule; sl_extra = shortlist
+// offset from the precision's built-in rule); '#' starts a comment. Loaded at the first search from $KNNS_TUNING_TABLE, else from
 // <directory of libknns_amd.so>/../tuning/gfx950.tsv; knns_set_tuning_table replaces it.
 namespace {
 struct TuneEntry {
@@ -188,10 +189,10 @@ knns_status load_tuning(const char* path, std::vector<TuneEntry>* out) {
     if (*p == '\0' || *p == '\n' || *p == '\r') continue;
     char prec[16];
     TuneEntry e{};
-    if (std::sscanf(p, "%15s %d %d %ld %ld %d %d %d", prec, &e.d, &e.k, &e.n, &e.m, &e.t.min_waves, &e.t.ratio,
-                    &e.t.qb) != 8) {
+    if (std::sscanf(p, "%15s %d %d %ld %ld %d %d %d %d", prec, &e.d, &e.k, &e.n, &e.m, &e.t.min_waves, &e.t.ratio,
+                    &e.t.qb, &e.t.sl_extra) != 9) {
       std::fclose(f);
-      return fail(KNNS_ERR_INVALID, "%s:%d: expected 'precision d k n m min_waves ratio qb'", path, ln);
+      return fail(KNNS_ERR_INVALID, "%s:%d: expected 'precision d k n m min_waves ratio qb sl_extra'", path, ln);
     }
     if (!std::strcmp(prec, "fp32")) e.prec = KNNS_PREC_FP32;
     else if (!std::strcmp(prec, "bf16")) e.prec = KNNS_PREC_BF16;
@@ -202,7 +203,8 @@ knns_status load_tuning(const char* path, std::vector<TuneEntry>* out) {
       return fail(KNNS_ERR_INVALID, "%s:%d: unknown precision '%s'", path, ln, prec);
     }
     if (e.d < 1 || e.k < 1 || e.n < 1 || e.m < 1 || e.t.min_waves < 1 || e.t.min_waves > 64 ||
-        (e.t.ratio != 0 && (e.t.ratio < 2 || e.t.ratio > 16)) || e.t.qb < 0 || e.t.qb > 4) {
+        (e.t.ratio != 0 && (e.t.ratio < 2 || e.t.ratio > 16)) || e.t.qb < 0 || e.t.qb > 4 ||
+        e.t.sl_extra < -1024 || e.t.sl_extra > 1024) {
       std::fclose(f);
       return fail(KNNS_ERR_INVALID, "%s:%d: value out of range", path, ln);
     }
@@ -261,23 +263,29 @@ void dims_for(int d, int* dp, int* dc) {
   *dp = d16 <= 128 ? *dc : (int)round_up(d, 128);
 }
 
-// Shortlist sizes K' per precision. Exactness does not depend on them (the certificate re-searches
-// any query whose shortlist it cannot prove complete); they set how often that happens. The
-// certificate needs the gap between the k-th exact distance and the K'-th approximate key to exceed
-// the key error bound, which grows with the operand rounding. Measured on a 1M x 128 mixture
-// corpus with 10k queries: BF16 at k = 100 re-searches 861 queries with K' = 266, 16 with 296 and 1
-// with 316; at k = 10, 333 with 61, 15 with 76, 1 with 90; at k = 1, 3 with 33 and none with 66.
-// FP32 at k + k/4 + 16 re-searches none at k = 10, 100 (k + 16 re-searched 1,465 at k = 100).
+// Shortlist sizes K' per precision: the K'-th approximate key sets each threshold. Exactness does
+// not depend on them: every threshold also keeps the certificate band above the K'-th key (rows
+// whose distance the key error bound cannot separate from it, BandArgs), and the certificate
+// re-searches any query whose final shortlist it cannot prove complete. A fixed margin instead of
+// the band needs K' far beyond k on dense data (1M x 128 mixture, k = 100: K' = 296 for all but 16
+// of 10k queries at BF16; 10M rows: none at 296). FP8, the approximate form, has no band: its K'
+// sets its recall.
 int default_shortlist(int prec, int k) {
   switch (prec) {
-    case KNNS_PREC_BF16: return k + 80 + (k >= 32 ? k + 16 : k / 2);
+    case KNNS_PREC_BF16: return k + k / 2 + 16;
     case KNNS_PREC_FP8: return 4 * k + 32;
-    default: return k + k / 4 + 16;
+    default: return k + 16;
   }
 }
 
+int shortlist_for(int prec, int k, int requested, int sl_extra, long m_eff) {
+  int cl = requested > 0 ? std::max(requested, k) : std::max(k, default_shortlist(prec, k) + sl_extra);
+  cl = (int)std::min<long>(cl, m_eff);
+  return std::max(std::min(cl, 2048), 1);
+}
+
 Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int ratio_req,
-               int slices_req, int min_waves, bool allow_dense, int qb_req) {
+               int slices_req, int min_waves, bool allow_dense, int qb_req, bool band) {
   Plan P{};
   dims_for(d, &P.dp, &P.dc);
   P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : (prec == KNNS_PREC_FP8 ? PREC_FP8 : PREC_F32);
@@ -289,6 +297,8 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   P.n_pad = P.groups * qw;
   P.kmax = kmax;
   P.cmax = cmax;
+  // the band keeps rows beyond the K'-th: room for up to 4 K' (at most 1024, the rerank's wave form)
+  P.carry_cap = band ? std::max(cmax, std::min(1024, 4 * cmax)) : cmax;
   // scan workgroups: 4 query groups each, 2 (or 1 for QB >= 3) resident per CU. Choose the slice count so that the
   // workgroups of a large round fill whole waves of resident workgroups (a last wave that is
   // mostly empty leaves the chip idle): the smallest S with >= min_waves waves and >= 95% of the
@@ -367,7 +377,7 @@ SearchSizes search_sizes(const knns_corpus_s* c, const Plan& P, long n, long tot
   z.self = excl ? (size_t)n_pad * 4 : 0;
   z.sub = (size_t)n_pad * P.slices_max * 2 * P.sub_cap * 8;
   z.subcnt = (size_t)n_pad * P.slices_max * 2 * 4;
-  z.carry = (size_t)n * P.cmax * 8;
+  z.carry = (size_t)n * P.carry_cap * 8;
   z.ccnt = (size_t)n * 4;
   z.flags = 16 * 4;
   z.odist = (size_t)total_out * (out64 ? 8 : 4);
@@ -559,6 +569,14 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   KNNS_TRY(launch_fill_i32((int*)B.flags, 0, 16, st));
 
   const int pk = P.prec_kernel;
+  BandArgs band{};
+  band.on = L.band ? 1 : 0;
+  band.qn2 = (const double*)B.qn2;
+  band.qrho = (const double*)B.qrho;
+  band.bnd = c->bnd;
+  band.bslot = pk;
+  band.gamma = key_gamma(P);
+  if (band.on && (!B.qn2 || !c->bnd)) return hipErrorInvalidValue;
   const void* xop = pk == PREC_BF16 ? (const void*)c->x16 : (pk == PREC_FP8 ? (const void*)c->x8 : (const void*)c->x32);
   const void* qop = pk == PREC_BF16 ? B.q16 : (pk == PREC_FP8 ? B.q8 : B.q32);
   const long ldx = pk == PREC_BF16 ? c->ld16 : (pk == PREC_FP8 ? c->ld8 : c->ld32);
@@ -586,6 +604,8 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     sa.flags = (int*)B.flags;
     sa.dense = R.dense;
     sa.n_dev = L.n_dev;
+    sa.band = band;
+    sa.thr_w = (uint32_t*)B.thr;
     // an overlapped upload: the scan reads up to one stage (<= 128 rows) past its round's end
     if (L.feed) KNNS_TRY(feed_rows(*L.feed, R.end + 128, st));
     if (hook) (*hook)(0);
@@ -597,7 +617,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     ma.carry_in_cnt = (const int*)B.ccnt[cur];
     ma.carry_out = (uint64_t*)B.carry[cur ^ 1];
     ma.carry_out_cnt = (int*)B.ccnt[cur ^ 1];
-    ma.carry_cap = P.cmax;
+    ma.carry_cap = P.carry_cap;
     ma.sub = (const uint64_t*)B.sub;
     ma.sub_cnt = (const int*)B.subcnt;
     ma.sub_cap = P.sub_cap;
@@ -611,12 +631,14 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     // seen) candidates
     {
       const long seen = R.begin - P.rounds[0].begin;
-      const double est = seen <= 0 ? (double)(R.end - R.begin) : (double)L.cmax * (double)(R.end - P.rounds[0].begin) / (double)seen;
+      const double est = (seen <= 0 ? (double)(R.end - R.begin) : (double)L.cmax * (double)(R.end - P.rounds[0].begin) / (double)seen) +
+                         (L.band ? 0.5 * L.cmax : 0.0);
       ma.wide = est > 900.0 ? 1 : 0;
     }
     ma.flags = (int*)B.flags;
     ma.self_ids = (const int*)B.self;
     ma.n_dev = L.n_dev;
+    ma.band = band;
     KNNS_TRY(launch_merge(ma, st));
     if (hook) (*hook)(2);
     cur ^= 1;
@@ -625,7 +647,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   FinalArgs fa{};
   fa.carry = (const uint64_t*)B.carry[cur];
   fa.carry_cnt = (const int*)B.ccnt[cur];
-  fa.carry_cap = P.cmax;
+  fa.carry_cap = P.carry_cap;
   fa.kc = (const int*)B.kc;
   fa.kq = (const int*)B.kq;
   fa.out_off = (const long*)B.off;
@@ -855,11 +877,12 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   // per-query k and shortlist
   const bool uniform = kq_host == nullptr;
   const long m_eff = excl ? std::max<long>(m - 1, 1) : m;
-  auto shortlist_of = [&](int k) {
-    int cl = p->shortlist > 0 ? std::max(p->shortlist, k) : default_shortlist(prec, k);
-    cl = (int)std::min<long>(cl, m_eff);
-    return std::max(std::min(cl, 2048), 1);
-  };
+  // the tuning entry of the search (its largest k), then the shortlists
+  int kmax_all = uniform ? kuni : 1;
+  if (!uniform)
+    for (long i = 0; i < n; ++i) kmax_all = std::max(kmax_all, kq_host[i]);
+  const Tuning tu = tuning_for(prec, c->d, std::min(std::max(kmax_all, 1), 1024), n, m);
+  auto shortlist_of = [&](int k) { return shortlist_for(prec, k, p->shortlist, tu.sl_extra, m_eff); };
   std::vector<int> kq, kc;
   std::vector<long> off;
   long total_out = 0;
@@ -885,9 +908,9 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     }
   }
 
-  const Tuning tu = tuning_for(prec, c->d, kmax, n, m);
+  const bool band = (p->flags & KNNS_NO_CERTIFY) == 0 && prec != KNNS_PREC_FP8;
   Plan P = make_plan(m, c->d, n, kmax, cmax, prec, E.cus, p->ratio > 0 ? p->ratio : tu.ratio, p->slices,
-                     tu.min_waves, true, p->qb > 0 ? p->qb : tu.qb);
+                     tu.min_waves, true, p->qb > 0 ? p->qb : tu.qb, band);
   for (Round& r : P.rounds) {   // planned over the range's rows, from its first row
     r.begin += row_begin;
     r.end += row_begin;
@@ -984,6 +1007,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   L.bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
   L.feed = feed;
   L.certify = (p->flags & KNNS_NO_CERTIFY) == 0;
+  L.band = band;
   // per-launch events (KNNS_KERNEL_TIMES): pairs (scan begin, scan end), (scan end, merge end),
   // (final begin, final end)
   std::vector<std::pair<size_t, size_t>> scan_ev, sel_ev;
@@ -1084,6 +1108,7 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
   S.dc = P.dc;
   S.dp = P.dp;
   S.uncertified = flags_h[2];
+  S.shortlist = cmax;
   S.researched = researched;
   S.research_s = research_s;
   E.last = S;
@@ -1190,17 +1215,17 @@ knns_status do_search_noalloc(knns_corpus c, const void* queries, int dtype, lon
   if (knns_status st = scratch_for_device(dev, &base, &cus)) return st;
   const long m = c->m;
   const long m_eff = excl ? std::max<long>(m - 1, 1) : m;
-  int cmax = p->shortlist > 0 ? std::max(p->shortlist, k) : default_shortlist(prec, k);
-  cmax = std::max(std::min((int)std::min<long>(cmax, m_eff), 2048), 1);
+  const Tuning tu = tuning_for(prec, c->d, k, n, m);
+  const int cmax = shortlist_for(prec, k, p->shortlist, tu.sl_extra, m_eff);
+  const bool band = (p->flags & KNNS_NO_CERTIFY) == 0 && prec != KNNS_PREC_FP8;
   // the planner's slice count, lowered until the workspace fits the scratch array (the answer does
   // not depend on the slice count, only the parallelism of the scan does)
-  const Tuning tu = tuning_for(prec, c->d, k, n, m);
   int sreq = p->slices;
   Plan P;
   SearchSizes z;
   for (;;) {
     P = make_plan(m, c->d, n, k, cmax, prec, cus, p->ratio > 0 ? p->ratio : tu.ratio, sreq, tu.min_waves, true,
-                  p->qb > 0 ? p->qb : tu.qb);
+                  p->qb > 0 ? p->qb : tu.qb, band);
     z = search_sizes(c, P, n, n * (long)k, excl, out64);
     if (carve_bytes(z) <= (size_t)KNNS_NOALLOC_SCRATCH_BYTES) break;
     if (P.slices_max <= 1)
@@ -1254,6 +1279,7 @@ knns_status do_search_noalloc(knns_corpus c, const void* queries, int dtype, lon
   L.out64 = out64;
   L.bitonic = p->select == KNNS_SELECT_BITONIC ? 1 : 0;
   L.certify = (p->flags & KNNS_NO_CERTIFY) == 0;
+  L.band = band;
   // the flags (overflow counter) are cleared by a fill kernel inside the sequence
   KNNS_HIP(enqueue_search(L, B, st, nullptr));
   int flags_h[16] = {0};
@@ -1491,8 +1517,8 @@ knns_status knns_search_once(const void* data, int dtype, long m, int d, const v
 knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns_stats* out) {
   if (!out || m < 1 || d < 1 || n < 1 || k < 1 || k > 1024)
     return fail(KNNS_ERR_INVALID, "bad plan arguments");
-  const int cl = std::min<long>(std::min(default_shortlist(precision, k), 2048), m);
   const Tuning tu = tuning_for(precision, d, k, n, m);
+  const int cl = shortlist_for(precision, k, 0, tu.sl_extra, m);
   Plan P = make_plan(m, d, n, k, cl, precision, 256, tu.ratio, 0, tu.min_waves, true, tu.qb);
   std::memset(out, 0, sizeof(*out));
   out->rounds = (long)P.rounds.size();
@@ -1501,6 +1527,7 @@ knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns
   out->qb = P.qb;
   out->dc = P.dc;
   out->dp = P.dp;
+  out->shortlist = cl;
   return KNNS_OK;
 }
 

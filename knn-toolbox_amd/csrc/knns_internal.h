@@ -30,6 +30,21 @@ constexpr uint32_t ORD_NEG_INF = 0x007fffffu;  // f2ord(-inf)
 inline int aug_f32() { return 4; }
 inline int aug_bf16() { return 16; }
 
+// Certificate band of the shortlist thresholds. With band on, a threshold computed from a K'-th
+// approximate key tau is f(tau) = tau + D(tau) rho + rho^2 / 2 + 2 alpha, where D(tau) = sqrt(2 tau +
+// ||q~||^2) is the approximate distance of a key-tau row, rho = ||q~ - q|| + max ||r~ - r|| and alpha
+// the accumulation bound: every row within the error band of the K'-th row is kept, so the final
+// certificate (k-th exact distance vs the last threshold) holds without a fixed shortlist margin.
+// f is increasing in tau, so thresholds stay non-increasing over rounds.
+struct BandArgs {
+  int on;
+  const double* qn2;     // ||q~||^2 per query slot
+  const double* qrho;    // ||q~ - q|| per query slot
+  const uint32_t* bnd;   // corpus maxima (PackArgs::bnd)
+  int bslot;             // operand slot in bnd
+  double gamma;          // key_gamma of the plan
+};
+
 struct ScanArgs {
   const void* X;         // packed corpus rows
   const void* Qp;        // packed query rows
@@ -51,6 +66,9 @@ struct ScanArgs {
                          // (query, slice, half 0) sub-pool (slice_rows <= sub_cap), fills = rows
   const int* n_dev;      // optional device query count: workgroups whose queries all lie past it
                          // return at once (graph-captured searches over a padded slot count)
+  BandArgs band;         // in-scan compaction keeps the band above its K'-th entry; a truncation
+                         // lowers thr[q] (atomicMin) so the merge never raises it past dropped rows
+  uint32_t* thr_w;       // thr, writable (the same array)
 };
 
 struct MergeArgs {
@@ -72,6 +90,7 @@ struct MergeArgs {
   int* flags;
   const int* self_ids;   // optional: corpus row excluded for each query (all-kNN), -1 = none
   const int* n_dev;      // optional: real query count on the device (overrides n)
+  BandArgs band;         // keep every entry below f(K'-th key) (up to carry_cap), threshold f(K'-th)
 };
 
 struct FinalArgs {

@@ -70,6 +70,28 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane) {
 }
 __device__ __forceinline__ int bits64(uint64_t x) { return x ? 64 - __clzll(x) : 0; }
 
+__device__ __forceinline__ double next_up_f32(double x) {   // a float strictly above x
+  const float f = (float)x;
+  float g = (double)f > x ? f : __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) + (f >= 0.f ? 1u : -1u));
+  if (g == 0.f) g = 0x1p-149f;
+  return (double)g;
+}
+
+// Threshold of the certificate band (BandArgs) for a K'-th key of order image t: the order image
+// of a float strictly above f(tau) = tau + D(tau) rho + rho^2 / 2 + 2 alpha; never below t.
+__device__ __forceinline__ uint32_t band_thr(const BandArgs& b, int q, uint32_t t) {
+  if (t >= ORD_POS_INF) return t;
+  const double tau = (double)ord2f(t);
+  const double qn2 = b.qn2[q];
+  const double rmax = (double)__builtin_bit_cast(float, b.bnd[BND_RHO + b.bslot]);
+  const double nmax = (double)__builtin_bit_cast(float, b.bnd[BND_NRM + b.bslot]);
+  const double alpha = b.gamma * (__builtin_sqrt(qn2) * nmax + 0.5 * nmax * nmax);
+  const double rho = b.qrho[q] + rmax;
+  const double D = __builtin_sqrt(__builtin_fmax(0.0, 2.0 * tau + qn2));
+  const uint32_t r = f2ord((float)next_up_f32(tau + D * rho + 0.5 * rho * rho + 2.0 * alpha));
+  return r > t ? r : t;
+}
+
 // ------------------------------------------------------------------------------------------------
 // K-th smallest of N distinct 64-bit entries (radix select, 8 bits per level), by one wave
 // (WAVE) or by the whole block (blockDim multiple of 64, <= 256). get(i) returns entry i. Requires 1 <= K <= N.
@@ -426,6 +448,27 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
       auto raw = [&](int i) -> uint64_t { return i < n0 ? p0[i] : p1[i - n0]; };
       auto get = [&](int i) -> uint64_t { return pool_key(raw(i)); };
       const uint64_t E = group_kth<true>(get, N, K, lds);
+      // the band above the K-th entry is kept when it fits in half the pool; otherwise exactly the
+      // K entries, and thr[q] is lowered to their K-th key so that no later threshold of this
+      // search admits the rows dropped here
+      uint32_t T = (uint32_t)(E >> 32);
+      bool banded = false;
+      if (a.band.on) {
+        const uint32_t fT = band_thr(a.band, q, T);
+        int cnt = 0;
+        for (int base = 0; base < N; base += 64) {
+          const int i = base + lane;
+          bool keep = false;
+          if (i < N) { const uint64_t e = get(i); keep = (uint32_t)(e >> 32) < fT || e <= E; }
+          cnt += __popcll(__builtin_amdgcn_ballot_w64(keep));
+        }
+        if (cnt <= lim / 2) {
+          T = fT;
+          banded = true;
+        } else if (lane == 0) {
+          atomicMin(&a.thr_w[q], T);
+        }
+      }
       // in-place order-preserving compaction of the logical array [p0 | p1] (writes never
       // overtake reads: a chunk is read completely before any of its outputs is stored)
       int outpos = 0;
@@ -433,7 +476,11 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
         const int i = base + lane;
         uint64_t e = 0;
         bool keep = false;
-        if (i < N) { e = raw(i); keep = pool_key(e) <= E; }
+        if (i < N) {
+          e = raw(i);
+          const uint64_t pk = pool_key(e);
+          keep = pk <= E || (banded && (uint32_t)(pk >> 32) < T);
+        }
         const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
         const int pos = outpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32),
                                   __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
@@ -443,7 +490,6 @@ __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& s
       }
       const int k0 = outpos < n0 ? outpos : n0;
       const int k1 = outpos - k0;
-      const uint32_t T = (uint32_t)(E >> 32);
       if (lane == c) { st.cnt[qb] = k0; }
       if (lane == c + 32) { st.cnt[qb] = k1; }
       if ((lane & 31) == c) {
@@ -986,6 +1032,16 @@ __global__ __launch_bounds__(256, QB >= 3 ? 1 : 2) void scan_swz_kernel(ScanArgs
 constexpr int MERGE_MAX_SRC = 1 + 2 * 128;
 constexpr int MW_ENT = 1024;   // entries gathered to LDS per wave (8 KB)
 
+// New threshold of a merge whose K'-th entry is E: its key, or with the certificate band f(key),
+// never above the query's current threshold (which an in-scan truncation may have lowered).
+__device__ __forceinline__ uint32_t merge_theta(const MergeArgs& a, int q, uint64_t E) {
+  const uint32_t tE = (uint32_t)(E >> 32);
+  if (!a.band.on) return tE;
+  const uint32_t f = band_thr(a.band, q, tE);
+  const uint32_t cur = a.thr[q];
+  return f < cur ? f : (cur > tE ? cur : tE);
+}
+
 __device__ __forceinline__ void ce_u64(uint64_t& x, uint64_t& y, bool up) {
   const bool sw = (x > y) == up;
   const uint64_t lo = sw ? y : x;
@@ -1110,13 +1166,15 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
     return (uint32_t)e == self ? ~0ull : pool_key(e);
   };
   uint64_t* cout = a.carry_out + (long)q * a.carry_cap;
+  uint32_t theta = 0;
+  bool banded = false;   // keep the band below theta as well as the entries up to E
   auto compact = [&](auto get, uint64_t E) {
     int outpos = 0;
     for (int base = 0; base < N; base += 64) {
       const int i = base + lane;
       uint64_t e = 0;
       bool keep = false;
-      if (i < N) { e = get(i); keep = e <= E && e != ~0ull; }
+      if (i < N) { e = get(i); keep = e != ~0ull && (e <= E || (banded && (uint32_t)(e >> 32) < theta)); }
       const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
       const int pos = outpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
@@ -1168,23 +1226,53 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
       while (P < N) P <<= 1;
       for (int i = N + lane; i < P; i += 64) ent[i] = ~0ull;
       wave_bitonic_u64_r8(ent, P);   // P >= 64
-      for (int i = lane; i < K; i += 64) cout[i] = ent[i];
       E = ent[K - 1];
+      theta = merge_theta(a, q, E);
+      // sorted: the band below theta is a prefix; it extends the first K entries, up to the carry
       kept = K;
+      if (a.band.on) {
+        int c = 0;
+        for (int base = 0; base < N; base += 64)
+          c += __popcll(__builtin_amdgcn_ballot_w64(base + lane < N && ent[base + lane] != ~0ull &&
+                                                    (uint32_t)(ent[base + lane] >> 32) < theta));
+        kept = c > K ? c : K;
+        if (kept > a.carry_cap) {
+          kept = a.carry_cap;
+          const uint32_t tl = (uint32_t)(ent[kept - 1] >> 32);
+          theta = theta < tl ? theta : tl;
+        }
+      }
+      for (int i = lane; i < kept; i += 64) cout[i] = ent[i];
     } else {
       auto lget = [&](int i) -> uint64_t { return ent[i]; };
       E = group_kth<true>(lget, N, K, kth_all[wv]);
+      theta = merge_theta(a, q, E);
+      banded = a.band.on;
       kept = compact(lget, E);
     }
   } else {
     E = group_kth<true>(gget, N, K, kth_all[wv]);
+    theta = merge_theta(a, q, E);
+    if (a.band.on) {
+      banded = true;
+      int c = 0;
+      for (int base = 0; base < N; base += 64) {
+        const uint64_t e = base + lane < N ? gget(base + lane) : ~0ull;
+        c += __popcll(__builtin_amdgcn_ballot_w64(e != ~0ull && (e <= E || (uint32_t)(e >> 32) < theta)));
+      }
+      if (c > a.carry_cap) {   // the band overflows the carry: keep its first carry_cap entries
+        E = group_kth<true>(gget, N, a.carry_cap, kth_all[wv]);
+        theta = theta < (uint32_t)(E >> 32) ? theta : (uint32_t)(E >> 32);
+        banded = false;
+      }
+    }
     kept = compact(gget, E);
   }
   if (lane == 0) {
     if (kept > a.carry_cap) kept = a.carry_cap;
     a.carry_out_cnt[q] = kept;
-    a.thr[q] = (uint32_t)(E >> 32);
-    if (kept != K) atomicAdd(&a.flags[0], 1);
+    a.thr[q] = theta;
+    if (kept < K || (!a.band.on && kept != K)) atomicAdd(&a.flags[0], 1);
   }
 }
 
@@ -1351,6 +1439,8 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
   };
   uint64_t* cout = a.carry_out + (long)q * a.carry_cap;
   uint64_t E = ~0ull;
+  uint32_t theta = 0;   // the new threshold (set when N > K)
+  bool banded = a.band.on;
   int kept = 0;
   if (N > 64 * U) {
     // large union (only with unusual shortlists or ratios): radix select over global memory
@@ -1363,11 +1453,22 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
       return key_of(lo, i - off[lo]);
     };
     E = group_kth<true>(gget, N, K, kth_all[wv]);
+    theta = merge_theta(a, q, E);
+    auto keep_of = [&](uint64_t e) { return e != ~0ull && (e <= E || (a.band.on && (uint32_t)(e >> 32) < theta)); };
+    int cnt = 0;
+    if (a.band.on)
+      for (int base = 0; base < N; base += 64)
+        cnt += __popcll(__builtin_amdgcn_ballot_w64(base + lane < N && keep_of(gget(base + lane))));
+    if (cnt > a.carry_cap) {   // the band overflows the carry: keep its first carry_cap entries
+      E = group_kth<true>(gget, N, a.carry_cap, kth_all[wv]);
+      theta = theta < (uint32_t)(E >> 32) ? theta : (uint32_t)(E >> 32);
+      banded = false;
+    }
     for (int base = 0; base < N; base += 64) {
       const int i = base + lane;
       uint64_t e = 0;
       bool keep = false;
-      if (i < N) { e = gget(i); keep = e <= E && e != ~0ull; }
+      if (i < N) { e = gget(i); keep = banded ? keep_of(e) : (e <= E && e != ~0ull); }
       const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
       const int pos = kept + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
       if (keep && pos < a.carry_cap) cout[pos] = e;
@@ -1413,14 +1514,33 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
       v[u] = src[u] < 0 ? ~0ull
              : (src[u] == 0 ? e : ((uint32_t)e == self ? ~0ull : pool_key(e)));
     }
-    if (N > K) E = wave_kth_regs<U>(v, N, K, kth_all[wv]);
+    if (N > K) {
+      E = wave_kth_regs<U>(v, N, K, kth_all[wv]);
+      theta = merge_theta(a, q, E);
+      if (banded) {
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (64 * u >= N) break;   // wave-uniform
+          const uint64_t e = v[u];
+          cnt += __popcll(__builtin_amdgcn_ballot_w64(lane + 64 * u < N && e != ~0ull &&
+                                                      (e <= E || (uint32_t)(e >> 32) < theta)));
+        }
+        if (cnt > a.carry_cap) {   // the band overflows the carry: keep its first carry_cap entries
+          E = wave_kth_regs<U>(v, N, a.carry_cap, kth_all[wv]);
+          theta = theta < (uint32_t)(E >> 32) ? theta : (uint32_t)(E >> 32);
+          banded = false;
+        }
+      }
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (64 * u >= N) break;   // wave-uniform
-      const bool keep = lane + 64 * u < N && v[u] <= E && v[u] != ~0ull;
+      const uint64_t e = v[u];
+      const bool keep = lane + 64 * u < N && e != ~0ull && (e <= E || (banded && (uint32_t)(e >> 32) < theta));
       const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
       const int pos = kept + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
-      if (keep && pos < a.carry_cap) cout[pos] = v[u];
+      if (keep && pos < a.carry_cap) cout[pos] = e;
       kept += __popcll(km);
     }
   }
@@ -1428,8 +1548,8 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
     if (kept > a.carry_cap) kept = a.carry_cap;
     a.carry_out_cnt[q] = kept;
     if (N > K) {
-      a.thr[q] = (uint32_t)(E >> 32);
-      if (kept != K) atomicAdd(&a.flags[0], 1);
+      a.thr[q] = theta;
+      if (kept < K || (!a.band.on && kept != K)) atomicAdd(&a.flags[0], 1);
     }
   }
 }
@@ -1446,12 +1566,6 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
 // nearest: the query is certified. Otherwise the kernel leaves in thr[q] the smallest key threshold
 // that certifies it -- every row with a key at or above it is provably farther than D_k -- for a
 // re-search that admits every row below it.
-__device__ __forceinline__ double next_up_f32(double x) {   // a float strictly above x
-  const float f = (float)x;
-  float g = (double)f > x ? f : __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) + (f >= 0.f ? 1u : -1u));
-  if (g == 0.f) g = 0x1p-149f;
-  return (double)g;
-}
 
 __device__ void certify_query(const FinalArgs& a, int q, int k, int Kq, double dk) {
   const uint32_t t = a.thr[q];
@@ -1513,7 +1627,7 @@ __global__ __launch_bounds__(256) void final_block_kernel(FinalArgs a) {
   // the merge leaves exactly the shortlist in the carry (at most K entries): rerank all of them
   for (int i = tid; i < cnt; i += nt) ek[i] = src[i];
   __syncthreads();
-  const int Kq = K < cnt ? K : cnt;
+  const int Kq = cnt;   // every carry entry: the shortlist and its certificate band
   // query master row
   const int d = a.d;
   const bool dq = d <= 4096;
@@ -1576,7 +1690,7 @@ __global__ __launch_bounds__(256) void final_block_kernel(FinalArgs a) {
 
 // Final rerank, one wave per query (shortlists up to FW_MAX). Each lane reranks up to four
 // candidates at a time: four independent FMA chains, each in the canonical sequential order.
-constexpr int FW_MAX = 512;
+constexpr int FW_MAX = 1024;
 
 __device__ __forceinline__ void ce_pair(uint64_t& xk, uint32_t& xv, uint64_t& yk, uint32_t& yv, bool up) {
   const bool gt = (xk > yk) || (xk == yk && xv > yv);
@@ -1676,7 +1790,7 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
   int cnt = a.carry_cnt[q];
   if (cnt > FW_MAX) cnt = FW_MAX;
   const int K = a.kc[q];
-  const int Kq = K < cnt ? K : cnt;
+  const int Kq = cnt;   // every carry entry: the shortlist and its certificate band
   const int k = a.kq[q];
   const int d = a.d;
   const uint64_t* src = a.carry + (long)q * a.carry_cap;

@@ -122,11 +122,8 @@ struct knns_server_s {
   Clock::time_point t_first_launch;
   bool any_launch = false;
 
-  int shortlist_of(int k) const {
-    int cl = knns::default_shortlist(prec, k);
-    cl = (int)std::min<long>(cl, c->m);
-    return std::max(std::min(cl, 2048), 1);
-  }
+  int sl_extra = 0;     // the tuning table's shortlist offset for this corpus and max_k
+  int shortlist_of(int k) const { return knns::shortlist_for(prec, k, 0, sl_extra, c->m); }
   long max_cap() const { return buckets.empty() ? 0 : buckets.back().cap; }
 
   void launch(Lane& L);
@@ -414,6 +411,7 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
   S->prec = prec;
   S->esz = c->dtype == KNNS_F64 ? 8 : 4;
   S->osz = c->dtype == KNNS_F64 ? 8 : 4;
+  S->sl_extra = knns::tuning_for(prec, c->d, cfg->max_k, cfg->max_batch, c->m).sl_extra;
   S->cmax = S->shortlist_of(cfg->max_k);
   // bucket capacities 64 * 4^j (and max_batch itself)
   std::vector<long> caps;
@@ -426,7 +424,7 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
   for (long cap : caps) {
     const knns::Tuning tu = knns::tuning_for(prec, c->d, cfg->max_k, cap, c->m);
     knns::Plan P = knns::make_plan(c->m, c->d, cap, cfg->max_k, S->cmax, prec, cus, tu.ratio, 0, tu.min_waves,
-                                   true, tu.qb);
+                                   true, tu.qb, prec != KNNS_PREC_FP8);
     const knns::SearchSizes y = knns::search_sizes(c, P, cap, cap * (long)cfg->max_k, false, S->osz == 8);
     knns::SearchSizes t = z;
 #define SRV_MAX(f) t.f = std::max(t.f, y.f)
@@ -505,6 +503,7 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
       SL.out64 = S->osz == 8;
       SL.n_dev = L.n_dev;
       SL.certify = true;
+      SL.band = prec != KNNS_PREC_FP8;
       hipGraph_t g = nullptr;
       hipGraphExec_t ex = nullptr;
       hipError_t e = hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal);

@@ -51,7 +51,8 @@ class Stats(ctypes.Structure):
                 ("rounds", ctypes.c_long), ("slow_compactions", ctypes.c_long), ("candidates", ctypes.c_long),
                 ("groups", ctypes.c_int), ("slices_max", ctypes.c_int), ("qb", ctypes.c_int),
                 ("dc", ctypes.c_int), ("dp", ctypes.c_int), ("uncertified", ctypes.c_long),
-                ("researched", ctypes.c_long), ("research_s", ctypes.c_double)]
+                ("researched", ctypes.c_long), ("research_s", ctypes.c_double),
+                ("shortlist", ctypes.c_long)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -80,7 +80,7 @@ def test_native_params_layout(knns_mod):
     assert (p.precision, p.k, p.select, p.shortlist, p.flags) == (1, 7, 0, 0, 0)
     # layout of knns_params / knns_stats as gcc lays them out from include/knns_native.h
     assert ctypes.sizeof(knns_mod.Params) == 56 and knns_mod.Params.qb.offset == 48
-    assert ctypes.sizeof(knns_mod.Stats) == 96 and knns_mod.Stats.research_s.offset == 88
+    assert ctypes.sizeof(knns_mod.Stats) == 104 and knns_mod.Stats.shortlist.offset == 96
 
 
 def test_planner_shapes(knns_mod):

@@ -8,14 +8,18 @@ radii rho_q = ||q~ - q||, rho_r = max ||r~ - r||,
 
     ||q - r|| >= sqrt(2 (tau - alpha) + ||q~||^2) - rho_q - rho_r  >  D_k.
 
-Uncertified queries are searched again with every row inside that bound admitted.
+Every threshold keeps the certificate band above its K'-th key (rows the bound cannot separate
+from it), so the shortlist grows where neighbours are dense; a query whose band overflows the
+carried shortlist is searched again with every row inside the bound admitted.
 
 CPU: the bound is a true lower bound on vectors whose rounding errors are aligned against the
 distance (the worst case of the triangle inequality), and leaving out either radius breaks it.
-GPU: on a near-tie shell (2,000 rows within 1e-3 of the same squared distance) the BF16 shortlist
-of k + k/2 + 16 provably misses true neighbours without the certificate, and the certified
-search equals the FP64 oracle for FP32, BF16 and FP64, through the native API, the packed
-batches, the no-allocation entry point and the drop-in.
+GPU: on a near-tie shell (333 rows per query within 1e-3 of the same squared distance) the BF16
+shortlist of k + k/2 + 16 provably misses true neighbours without the certificate, while the
+certified search (band kept in one pass) equals the FP64 oracle for FP32, BF16 and FP64, through
+the native API, the packed batches, the no-allocation entry point and the drop-in; on a denser
+shell (1,500 rows per query, beyond the carried shortlist) the certificate fails, the re-search
+runs and the answer is again the oracle's.
 """
 import numpy as np
 import pytest
@@ -112,12 +116,25 @@ def test_near_tie_shell_is_certified_exact(gpu, knns_mod, shell, prec):
             Dg, Ig = c.search(Q, k, precision=prec)
             st = knns_mod.last_stats()
             assert np.array_equal(Ig, I[:, :k]) and np.array_equal(Dg, D[:, :k].astype(np.float32))
-            assert st["uncertified"] > 0 and st["researched"] == st["uncertified"]
+            assert st["researched"] == st["uncertified"]
             if prec == "bf16":
                 # without the certificate the k + k/2 + 16 BF16 shortlist misses true neighbours
                 Dn, In = c.search(Q, k, precision=prec, certify=False)
                 assert knns_mod.last_stats()["researched"] == 0
                 assert not np.array_equal(In, I[:, :k])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_band_overflow_is_researched(gpu, knns_mod, oracle_mod, prec):
+    R, Q = _shell(seed=9, n_shell=3000, n_far=20000, n_q=2)      # 1,500 shell rows per query
+    D, I = oracle_mod.exact_topk(Q, R, 100)
+    _case("dense_shell", prec=prec)
+    with knns_mod.Corpus(R, bf16=True) as c:
+        Dg, Ig = c.search(Q, 100, precision=prec)
+        st = knns_mod.last_stats()
+    assert np.array_equal(Ig, I) and np.array_equal(Dg, D.astype(np.float32))
+    assert st["uncertified"] > 0 and st["researched"] == st["uncertified"]
 
 
 @pytest.mark.gpu

@@ -3,6 +3,7 @@ parsing, nearest-entry lookup and its effect on the plan. CPU only (knns_plan_de
 device work)."""
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 TABLE = Path(__file__).resolve().parent.parent / "knn-toolbox_amd" / "tuning" / "gfx950.tsv"
@@ -22,10 +23,10 @@ def test_committed_table_parses(knns_mod, restore_table):
     assert {r[0] for r in rows} == {"bf16", "fp32", "fp8"}
     # every entry: d, k, n and the two knobs are integers in range, the time is positive
     for r in rows:
-        d, k, n, m, w, ratio, qb = map(int, r[1:8])
+        d, k, n, m, w, ratio, qb, extra = map(int, r[1:9])
         assert d > 0 and 1 <= k <= 1024 and n > 0 and m > 0 and w >= 1 and (ratio == 0 or ratio >= 2)
-        assert 0 <= qb <= 4
-        assert float(r[8]) > 0
+        assert 0 <= qb <= 4 and -1024 <= extra <= 1024
+        assert float(r[9]) > 0
 
 
 def test_table_drives_slices_and_ratio(knns_mod, tmp_path, restore_table):
@@ -35,7 +36,7 @@ def test_table_drives_slices_and_ratio(knns_mod, tmp_path, restore_table):
     slices = {}
     for w in (3, 5, 8):
         t = tmp_path / f"w{w}.tsv"
-        t.write_text(f"# test\nbf16 128 100 10000 1000000 {w} 0 0\n")
+        t.write_text(f"# test\nbf16 128 100 10000 1000000 {w} 0 0 0\n")
         assert L.knns_set_tuning_table(str(t).encode()) == 0
         slices[w] = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")["slices_max"]
     assert slices[3] < slices[5] < slices[8]
@@ -43,7 +44,7 @@ def test_table_drives_slices_and_ratio(knns_mod, tmp_path, restore_table):
     rounds = {}
     for r in (0, 2, 8):
         t = tmp_path / f"r{r}.tsv"
-        t.write_text(f"bf16 128 100 10000 1000000 5 {r} 0\n")
+        t.write_text(f"bf16 128 100 10000 1000000 5 {r} 0 0\n")
         assert L.knns_set_tuning_table(str(t).encode()) == 0
         rounds[r] = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")["rounds"]
     assert rounds[2] > rounds[0] > rounds[8]
@@ -52,25 +53,26 @@ def test_table_drives_slices_and_ratio(knns_mod, tmp_path, restore_table):
 def test_nearest_entry_and_precision(knns_mod, tmp_path, restore_table):
     L = knns_mod.native()
     t = tmp_path / "t.tsv"
-    t.write_text("bf16 128 100 10000 1000000 3 0 0\nbf16 128 100 256 1000000 8 0 0\nfp32 128 100 10000 1000000 8 0 0\n")
+    t.write_text("bf16 128 100 10000 1000000 3 0 0 0\nbf16 128 100 256 1000000 8 0 0 0\nfp32 128 100 10000 1000000 8 0 0 0\n")
     assert L.knns_set_tuning_table(str(t).encode()) == 0
     near_big = knns_mod.plan_describe(1_000_000, 120, 9000, 90, "bf16")["slices_max"]
     t2 = tmp_path / "t2.tsv"
-    t2.write_text("bf16 128 100 10000 1000000 3 0 0\n")
+    t2.write_text("bf16 128 100 10000 1000000 3 0 0 0\n")
     assert L.knns_set_tuning_table(str(t2).encode()) == 0
     assert knns_mod.plan_describe(1_000_000, 120, 9000, 90, "bf16")["slices_max"] == near_big
     # cleared table: built-in defaults (5 waves)
     assert L.knns_set_tuning_table(None) == 0
     t3 = tmp_path / "t3.tsv"
-    t3.write_text("bf16 128 100 10000 1000000 5 0 0\n")
+    t3.write_text("bf16 128 100 10000 1000000 5 0 0 0\n")
     default = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")["slices_max"]
     assert L.knns_set_tuning_table(str(t3).encode()) == 0
     assert knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")["slices_max"] == default
 
 
-@pytest.mark.parametrize("text,where", [("bf16 128 100\n", ":1:"), ("# c\nfp99 1 1 1 1 5 0 0\n", ":2: unknown precision"),
-                                        ("bf16 128 100 10 100 0 0 0\n", "out of range"), ("bf16 128 100 10 100 5 1 0\n", "out of range"),
-                                        ("bf16 128 100 10 100 5 0 7\n", "out of range"), ("bf16 128 100 10 0 5 0 0\n", "out of range")])
+@pytest.mark.parametrize("text,where", [("bf16 128 100\n", ":1:"), ("# c\nfp99 1 1 1 1 5 0 0 0\n", ":2: unknown precision"),
+                                        ("bf16 128 100 10 100 0 0 0 0\n", "out of range"), ("bf16 128 100 10 100 5 1 0 0\n", "out of range"),
+                                        ("bf16 128 100 10 100 5 0 7 0\n", "out of range"), ("bf16 128 100 10 0 5 0 0 0\n", "out of range"),
+                                        ("bf16 128 100 10 100 5 0 0 2000\n", "out of range")])
 def test_malformed_tables_rejected(knns_mod, tmp_path, restore_table, text, where):
     L = knns_mod.native()
     t = tmp_path / "bad.tsv"
@@ -78,3 +80,59 @@ def test_malformed_tables_rejected(knns_mod, tmp_path, restore_table, text, wher
     assert L.knns_set_tuning_table(str(t).encode()) == knns_mod.KNNS_ERR_INVALID
     assert where in L.knns_last_error().decode()
     assert L.knns_set_tuning_table(b"/nonexistent/table.tsv") == knns_mod.KNNS_ERR_INVALID
+
+
+def test_table_drives_tile_shape_and_shortlist(knns_mod, tmp_path, restore_table):
+    """The tile shape (query blocks per scan wave) and the shortlist come from the table; a shape
+    the scan is not built for falls back to the built-in rule; corpus size is part of the key."""
+    L = knns_mod.native()
+    got = {}
+    for qb in (1, 2, 3, 4):
+        t = tmp_path / f"qb{qb}.tsv"
+        t.write_text(f"bf16 128 100 10000 1000000 5 0 {qb} 0\n")
+        assert L.knns_set_tuning_table(str(t).encode()) == 0
+        got[qb] = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")
+    assert [got[q]["qb"] for q in (1, 2, 3, 4)] == [1, 2, 3, 4]
+    # fewer queries per workgroup row at smaller QB: more query groups
+    assert got[1]["groups"] > got[2]["groups"] > got[4]["groups"]
+    t = tmp_path / "fp32.tsv"
+    t.write_text("fp32 128 100 10000 1000000 5 0 4 0\n")          # FP32 has no 4-block scan
+    assert L.knns_set_tuning_table(str(t).encode()) == 0
+    assert knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "fp32")["qb"] == 1
+    # shortlist offset
+    base = None
+    for extra in (0, 40, -60):
+        t = tmp_path / f"sl{extra}.tsv"
+        t.write_text(f"bf16 128 100 10000 1000000 5 0 2 {extra}\n")
+        assert L.knns_set_tuning_table(str(t).encode()) == 0
+        sl = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")["shortlist"]
+        base = sl if extra == 0 else base
+        assert sl == base + extra
+    # corpus size as a key: two entries differing only in m
+    t = tmp_path / "m.tsv"
+    t.write_text("bf16 128 100 10000 1000000 5 0 1 0\nbf16 128 100 10000 10000000 5 0 2 0\n")
+    assert L.knns_set_tuning_table(str(t).encode()) == 0
+    assert knns_mod.plan_describe(1_200_000, 128, 10_000, 100, "bf16")["qb"] == 1
+    assert knns_mod.plan_describe(9_000_000, 128, 10_000, 100, "bf16")["qb"] == 2
+
+
+@pytest.mark.gpu
+def test_tile_shapes_give_identical_results(gpu, knns_mod, oracle_mod, tmp_path, restore_table):
+    """Every tile shape the table can select returns the same answer (the oracle's)."""
+    L = knns_mod.native()
+    R = oracle_mod.mixture(200000, 128, 128, 0.1, center_seed=61, point_seed=62)
+    Q = oracle_mod.mixture(700, 128, 128, 0.1, center_seed=61, point_seed=63)
+    Do, Io = oracle_mod.exact_topk(Q[:64], R, 50)
+    with knns_mod.Corpus(R, bf16=True) as c:
+        ref = None
+        for qb in (1, 2, 3, 4):
+            t = tmp_path / f"g{qb}.tsv"
+            t.write_text(f"bf16 128 50 700 200000 5 0 {qb} 0\n")
+            assert L.knns_set_tuning_table(str(t).encode()) == 0
+            print(f"\n[case] tile qb={qb}", flush=True)
+            D, I = c.search(Q, 50, precision="bf16")
+            assert knns_mod.last_stats()["qb"] == qb
+            assert np.array_equal(I[:64], Io) and np.array_equal(D[:64], Do.astype(np.float32))
+            if ref is None:
+                ref = (D, I)
+            assert np.array_equal(I, ref[1]) and np.array_equal(D, ref[0])
