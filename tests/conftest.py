@@ -43,3 +43,13 @@ def gpu(knns_mod):
     if n <= 0:
         pytest.fail("no GPU visible to the HIP runtime (GPU tests need an MI355X)")
     return n
+
+
+@pytest.fixture
+def builtin_plans(knns_mod):
+    """The planner's built-in rules (no tuning table); the package's table is restored afterwards."""
+    L = knns_mod.native()
+    assert L.knns_set_tuning_table(None) == 0
+    yield
+    table = ROOT / "knn-toolbox_amd" / "tuning" / "gfx950.tsv"
+    assert L.knns_set_tuning_table(str(table).encode()) == 0

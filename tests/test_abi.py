@@ -83,7 +83,7 @@ def test_native_params_layout(knns_mod):
     assert ctypes.sizeof(knns_mod.Stats) == 104 and knns_mod.Stats.shortlist.offset == 96
 
 
-def test_planner_shapes(knns_mod):
+def test_planner_shapes(knns_mod, builtin_plans):
     s = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")
     assert s["qb"] == 2 and s["dc"] == 128 and s["dp"] == 128
     assert s["groups"] == 160  # ceil(10000 / 64) rounded up to whole workgroups of 4 groups

@@ -184,7 +184,7 @@ def test_fp8_needs_fp8_corpus(gpu, knns_mod):
             c.search(R[:4], 3, precision="fp8")
 
 
-def test_fp8_plan_uses_fp8_row_bytes(knns_mod):
+def test_fp8_plan_uses_fp8_row_bytes(knns_mod, builtin_plans):
     # the FP8 scan's chunk is the FP8 row (d rounded up to 64, 128 or 256 bytes, the widths the
     # kernel is built for); the FP32 rerank rows keep their own padding
     for d, dc, dp in [(1, 64, 32), (24, 64, 32), (64, 64, 64), (65, 128, 96), (96, 128, 96), (128, 128, 128),
