@@ -101,8 +101,12 @@ def cpu_baseline(Q, R, k) -> dict:
                       f"of {nproc} CPUs on {model}"}
 
 
+SHORTLIST_RULE = ("per-request shortlist K' = k + k/2 + 16 plus the tuning table's offset for (d, k, batch, m), "
+                  "certified (error-bound band, re-search of what it cannot prove), exact FP32 rerank")
+
+
 def run_serving(knns, corpus, Q, seconds: float, fractions=(0.5,), max_batch: int = 4096,
-                corpus_desc: str = "1,000,000 x 128 BF16 (resident), shortlist k+80+(k<32 ? k/2 : k+16), certified, exact FP32 rerank",
+                corpus_desc: str = "1,000,000 x 128 BF16 (resident), " + SHORTLIST_RULE,
                 lanes: int = 0) -> dict:
     """Requests of log-uniform size in [1, 4096] queries (drawn from Q) with k uniform in
     {1, 10, 100}. Phase 1, closed loop: 8 requests kept in flight -> saturation q/s. Phase 2,
@@ -208,7 +212,7 @@ def run_c5(args, knns, workloads, torch) -> dict:
     log(f"C5 corpus resident in {time.time() - t0:.1f}s")
     m, d = R.shape
     del R
-    desc = f"{m:,} x {d} BF16 (resident, FP32 master), shortlist k+80+(k<32 ? k/2 : k+16), certified, exact FP32 rerank"
+    desc = f"{m:,} x {d} BF16 (resident, FP32 master), " + SHORTLIST_RULE
     # the batch cap trades throughput for tail latency: a batch of 16384 queries on this corpus
     # runs ~90 ms, and requests that arrive meanwhile wait for it
     runs = {mb: run_serving(knns, corpus, Q, args.serving_seconds, fractions=(0.25, 0.5, 0.75, 0.9),
@@ -264,7 +268,7 @@ def run_c3(args, knns, workloads, torch) -> dict:
         D = Dd.cpu().numpy()
         tf = 2.0 * n * m * d / st["scan_s"] / 1e12
         chk = cpu_check(Q[rs], R, k, I[rs], D[rs])
-        out[name] = {"qps": n / sec, "ms_per_step": sec * 1e3, "shortlist": sl or k + k // 4 + 16,
+        out[name] = {"qps": n / sec, "ms_per_step": sec * 1e3, "shortlist": st["shortlist"],
                      "scan_ms": st["scan_s"] * 1e3, "scan_tflops": tf, "scan_frac_of_peak": tf / peak[prec],
                      **{c: chk[c] for c in ("recall_at_k", "max_rel_dist_err", "indices_identical")}}
         log(f"C3 {name}: {out[name]}")
@@ -560,8 +564,7 @@ def main() -> int:
         m5 = R5.shape[0]
         del R5
         serving_c5 = run_serving(knns, c5, Q5, args.serving_seconds, fractions=(0.5,), max_batch=4096,
-                                 corpus_desc=f"{m5:,} x 128 BF16 (resident, FP32 master), shortlist 2k+96, "
-                                             "certified, exact FP32 rerank")
+                                 corpus_desc=f"{m5:,} x 128 BF16 (resident, FP32 master), " + SHORTLIST_RULE)
         c5.close()
         log(f"serving C5: {serving_c5}")
 
@@ -581,7 +584,7 @@ def main() -> int:
         "data": "synthetic (1024-centre Gaussian mixture, numpy PCG64 seeds 2/3)",
         "config": {"workload": "C2: corpus 1,000,000 x 128 FP32 (BF16 copy), 10,000 queries, k=100",
                    "m": m, "n": n, "d": d, "k": k,
-                   "shortlist": args.shortlist or (k + 80 + (k + 16 if k >= 32 else k // 2)), "parallelism": "single GPU"},
+                   "shortlist": stats["shortlist"], "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
                      "traffic": measured_traffic(m, n, k),

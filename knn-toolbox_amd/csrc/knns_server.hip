@@ -40,6 +40,19 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// A lane's stream. HIP pools its hardware queues: once a process holds GPU_MAX_HW_QUEUES streams
+// (4 by default: the null stream, the engine's streams and a caller's own count), a new stream
+// shares the least-used queue, and streams on one queue run strictly in order, so two lanes
+// could serialise. A stream created with a CU mask always gets a queue of its own; the mask
+// here enables every CU, so the lane's kernels see the whole device. (Such a stream is not
+// created non-blocking: it orders with work on the null stream, which the server never uses.)
+bool lane_stream_create(hipStream_t* s, int cus) {
+  std::vector<uint32_t> mask((size_t)(cus + 31) / 32, 0xFFFFFFFFu);
+  if (hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data()) == hipSuccess) return true;
+  (void)hipGetLastError();
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking) == hipSuccess;
+}
+
 struct Request {
   unsigned long long ticket = 0;
   std::vector<char> q;   // the request's queries (host copy)
@@ -122,8 +135,10 @@ struct knns_server_s {
   Clock::time_point t_first_launch;
   bool any_launch = false;
 
-  int sl_extra = 0;     // the tuning table's shortlist offset for this corpus and max_k
-  int shortlist_of(int k) const { return knns::shortlist_for(prec, k, 0, sl_extra, c->m); }
+  // K' per k (index k = 1..max_k): the built-in rule plus the tuning table's offset for that k
+  // on this corpus at max_batch queries, the shortlist a plain search of that k would use
+  std::vector<int> sl_k;
+  int shortlist_of(int k) const { return sl_k[k]; }
   long max_cap() const { return buckets.empty() ? 0 : buckets.back().cap; }
 
   void launch(Lane& L);
@@ -411,8 +426,13 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
   S->prec = prec;
   S->esz = c->dtype == KNNS_F64 ? 8 : 4;
   S->osz = c->dtype == KNNS_F64 ? 8 : 4;
-  S->sl_extra = knns::tuning_for(prec, c->d, cfg->max_k, cfg->max_batch, c->m).sl_extra;
-  S->cmax = S->shortlist_of(cfg->max_k);
+  S->sl_k.assign((size_t)cfg->max_k + 1, 1);
+  S->cmax = 1;
+  for (int k = 1; k <= cfg->max_k; ++k) {
+    const int extra = knns::tuning_for(prec, c->d, k, cfg->max_batch, c->m).sl_extra;
+    S->sl_k[k] = knns::shortlist_for(prec, k, 0, extra, c->m);
+    S->cmax = std::max(S->cmax, S->sl_k[k]);
+  }
   // bucket capacities 64 * 4^j (and max_batch itself)
   std::vector<long> caps;
   for (long cap = 64; cap < cfg->max_batch; cap *= 4) caps.push_back(cap);
@@ -477,7 +497,7 @@ knns_status knns_server_create(knns_server* out, knns_corpus c, const knns_serve
     if (!ok) break;
     L.h_q = (char*)hq; L.h_kq = (int*)hkq; L.h_kc = (int*)hkc; L.h_off = (long*)hoff;
     L.h_n = (int*)hn; L.h_flags = (int*)hfl; L.h_dist = (char*)hd; L.h_idx = (int32_t*)hi;
-    ok = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
+    ok = lane_stream_create(&L.stream, cus) &&
          hipEventCreate(&L.ev_start) == hipSuccess && hipEventCreate(&L.ev_end) == hipSuccess;
   }
   if (!ok) {

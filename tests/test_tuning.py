@@ -136,3 +136,30 @@ def test_tile_shapes_give_identical_results(gpu, knns_mod, oracle_mod, tmp_path,
             if ref is None:
                 ref = (D, I)
             assert np.array_equal(I, ref[1]) and np.array_equal(D, ref[0])
+
+
+@pytest.mark.gpu
+def test_server_shortlist_follows_each_k(gpu, knns_mod, oracle_mod, tmp_path, restore_table):
+    """A server's shortlist for a request of k is the one a plain search of that k would use: the
+    table's offset for max_k (here -32 at k = 100) is not applied to k = 1 requests, whose own
+    rule (k + 16, with the k = 10 entry's offset 0) certifies every query in one pass."""
+    L = knns_mod.native()
+    t = tmp_path / "srv.tsv"
+    t.write_text("bf16 128 100 4096 200000 5 0 2 -32\nbf16 128 10 4096 200000 5 0 2 0\n")
+    assert L.knns_set_tuning_table(str(t).encode()) == 0
+    R = oracle_mod.mixture(200000, 128, 128, 0.1, center_seed=71, point_seed=72)
+    Q = oracle_mod.mixture(3000, 128, 128, 0.1, center_seed=71, point_seed=73)
+    Do, Io = oracle_mod.exact_topk(Q[:32], R, 1)
+    with knns_mod.Corpus(R, bf16=True) as c:
+        print("\n[case] server k=1 with max_k=100", flush=True)
+        with knns_mod.Server(c, precision="bf16", max_k=100, max_batch=4096) as srv:
+            srv.stats(reset=True)
+            tickets = [srv.submit(Q[s:s + 500], 1) for s in range(0, 3000, 500)]
+            got = [srv.result(tk) for tk in tickets]
+            st = srv.stats()
+        assert st["researched"] == 0
+        D, I = np.concatenate([g[0] for g in got]), np.concatenate([g[1] for g in got])
+        assert np.array_equal(I[:32], Io) and np.array_equal(D[:32], Do.astype(np.float32))
+        Da, Ia = c.search(Q, 1, precision="bf16")
+        assert knns_mod.last_stats()["uncertified"] == 0
+        assert np.array_equal(Ia, I) and np.array_equal(Da, D)
