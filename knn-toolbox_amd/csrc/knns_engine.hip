@@ -627,13 +627,17 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     ma.kc = (const int*)B.kc;
     ma.n = (int)n;
     ma.bitonic = L.bitonic;
-    // expected union: the dense round's rows, else the carry plus about cmax * (new rows / rows
-    // seen) candidates
+    // expected union: the first round's rows (nothing is carried into it), else the carry (with
+    // the band's rows beyond K') plus about cmax * (new rows / rows seen) candidates
     {
       const long seen = R.begin - P.rounds[0].begin;
-      const double est = (seen <= 0 ? (double)(R.end - R.begin) : (double)L.cmax * (double)(R.end - P.rounds[0].begin) / (double)seen) +
-                         (L.band ? 0.5 * L.cmax : 0.0);
-      ma.wide = est > 900.0 ? 1 : 0;
+      if (seen <= 0) {
+        ma.wide = R.end - R.begin > 1024 ? 1 : 0;   // exact: at most one entry per row
+      } else {
+        const double est = (double)L.cmax * (double)(R.end - P.rounds[0].begin) / (double)seen +
+                           (L.band ? 0.5 * L.cmax : 0.0);
+        ma.wide = est > 900.0 ? 1 : 0;   // an estimate: keep a margin below the 1024 slots
+      }
     }
     ma.flags = (int*)B.flags;
     ma.self_ids = (const int*)B.self;
@@ -670,6 +674,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   fa.bnd = c->bnd;
   fa.bslot = P.prec_kernel;
   fa.gamma = key_gamma(P);
+  fa.prune = band.on;
   if (L.feed) KNNS_TRY(feed_rows(*L.feed, c->rows_pad, st));
   if (hook) (*hook)(3);
   KNNS_TRY(launch_final(fa, st));

@@ -124,6 +124,10 @@ struct FinalArgs {
   const uint32_t* bnd;   // corpus maxima (PackArgs::bnd); slot bslot
   int bslot;
   double gamma;          // relative bound of the scan's FP32 accumulation (all roundings of a key)
+  // prune = 1 (carries with a certificate band; needs qn2, qrho, bnd): the rerank first ranks the
+  // carry's first min(K', cnt) entries exactly, then skips every further entry whose key proves it
+  // farther than the k-th of those (the same bound as the certificate's)
+  int prune;
 };
 
 struct PackArgs {

@@ -110,7 +110,7 @@ __device__ __forceinline__ void group_sync() {
 
 // each(f) calls f(e) for every entry this thread holds (the entries of the group are the union).
 template <bool WAVE, class Each>
-__device__ uint64_t group_kth_each(Each each, int N, int K, uint32_t* lds) {
+__device__ __forceinline__ uint64_t group_kth_each(Each each, int N, int K, uint32_t* lds) {
   uint32_t* hist = lds;                          // 256 bins
   uint64_t* red = (uint64_t*)(lds + 256);        // 2 x (waves) u64, waves <= 4
   const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
@@ -202,7 +202,7 @@ __device__ uint64_t group_kth_each(Each each, int N, int K, uint32_t* lds) {
 }
 
 template <bool WAVE, class Get>
-__device__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
+__device__ __forceinline__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
   const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   const int nt = WAVE ? 64 : (int)blockDim.x;
   return group_kth_each<WAVE>([&](auto f) { for (int i = tid; i < N; i += nt) f(get(i)); }, N, K, lds);
@@ -1168,18 +1168,25 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a) {
   uint64_t* cout = a.carry_out + (long)q * a.carry_cap;
   uint32_t theta = 0;
   bool banded = false;   // keep the band below theta as well as the entries up to E
+  // the entries up to E first, then (banded) the band beyond E below theta: the rerank takes the
+  // carry's first K' entries as the shortlist proper
   auto compact = [&](auto get, uint64_t E) {
     int outpos = 0;
-    for (int base = 0; base < N; base += 64) {
-      const int i = base + lane;
-      uint64_t e = 0;
-      bool keep = false;
-      if (i < N) { e = get(i); keep = e != ~0ull && (e <= E || (banded && (uint32_t)(e >> 32) < theta)); }
-      const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
-      const int pos = outpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
-      if (keep && pos < a.carry_cap) cout[pos] = e;
-      outpos += __popcll(km);
+    for (int pass = 0; pass < (banded ? 2 : 1); ++pass) {
+      for (int base = 0; base < N; base += 64) {
+        const int i = base + lane;
+        uint64_t e = 0;
+        bool keep = false;
+        if (i < N) {
+          e = get(i);
+          keep = e != ~0ull && (pass == 0 ? e <= E : (e > E && (uint32_t)(e >> 32) < theta));
+        }
+        const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+        const int pos = outpos + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+        if (keep && pos < a.carry_cap) cout[pos] = e;
+        outpos += __popcll(km);
+      }
     }
     return outpos;
   };
@@ -1298,7 +1305,7 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // one pass of 32-bit arithmetic and two wave syncs; entries that share the K-th key are resolved
 // by the 64-bit select over that key's entries only. hist: >= 4 * 256 uint32 of LDS.
 template <int U>
-__device__ uint64_t wave_kth_regs(const uint64_t (&v)[U], int N, int K, uint32_t* hist) {
+__device__ __forceinline__ uint64_t wave_kth_regs(const uint64_t (&v)[U], int N, int K, uint32_t* hist) {
   const int lane = threadIdx.x & 63;
   uint32_t kmn = ~0u, kmx = 0u;
   uint64_t emn = ~0ull, emx = 0ull;
@@ -1464,15 +1471,21 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
       theta = theta < (uint32_t)(E >> 32) ? theta : (uint32_t)(E >> 32);
       banded = false;
     }
-    for (int base = 0; base < N; base += 64) {
-      const int i = base + lane;
-      uint64_t e = 0;
-      bool keep = false;
-      if (i < N) { e = gget(i); keep = banded ? keep_of(e) : (e <= E && e != ~0ull); }
-      const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
-      const int pos = kept + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
-      if (keep && pos < a.carry_cap) cout[pos] = e;
-      kept += __popcll(km);
+    // the entries up to E first, then (banded) the band beyond E
+    for (int pass = 0; pass < (banded ? 2 : 1); ++pass) {
+      for (int base = 0; base < N; base += 64) {
+        const int i = base + lane;
+        uint64_t e = 0;
+        bool keep = false;
+        if (i < N) {
+          e = gget(i);
+          keep = e != ~0ull && (pass == 0 ? e <= E : (e > E && keep_of(e)));
+        }
+        const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+        const int pos = kept + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+        if (keep && pos < a.carry_cap) cout[pos] = e;
+        kept += __popcll(km);
+      }
     }
   } else {
     // per-entry source map, then all loads of the lane in flight at once
@@ -1533,16 +1546,23 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
         }
       }
     }
+    // the entries up to E first, then (banded) the band beyond E below theta: the rerank takes
+    // the carry's first K' entries as the shortlist proper
+    auto emit = [&](auto band_pass) {   // band_pass: std::false_type, then std::true_type
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (64 * u >= N) break;   // wave-uniform
-      const uint64_t e = v[u];
-      const bool keep = lane + 64 * u < N && e != ~0ull && (e <= E || (banded && (uint32_t)(e >> 32) < theta));
-      const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
-      const int pos = kept + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
-      if (keep && pos < a.carry_cap) cout[pos] = e;
-      kept += __popcll(km);
-    }
+      for (int u = 0; u < U; ++u) {
+        if (64 * u >= N) break;   // wave-uniform
+        const uint64_t e = v[u];
+        const bool keep = lane + 64 * u < N && e != ~0ull &&
+                          (band_pass ? (e > E && (uint32_t)(e >> 32) < theta) : e <= E);
+        const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+        const int pos = kept + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+        if (keep && pos < a.carry_cap) cout[pos] = e;
+        kept += __popcll(km);
+      }
+    };
+    emit(std::false_type{});
+    if (banded && N > K) emit(std::true_type{});
   }
   if (lane == 0) {
     if (kept > a.carry_cap) kept = a.carry_cap;
@@ -1567,18 +1587,24 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
 // that certifies it -- every row with a key at or above it is provably farther than D_k -- for a
 // re-search that admits every row below it.
 
+// Every row whose approximate key exceeds the returned value lies provably farther from query q
+// than sqrt(dk), dk an exact squared distance of the rerank: the bound above solved for the key.
+__device__ double key_beyond(const FinalArgs& a, int q, double dk) {
+  const double qn2 = a.qn2[q], rq = a.qrho[q];
+  const double rmax = (double)__builtin_bit_cast(float, a.bnd[BND_RHO + a.bslot]);
+  const double nmax = (double)__builtin_bit_cast(float, a.bnd[BND_NRM + a.bslot]);
+  const double alpha = a.gamma * (__builtin_sqrt(qn2) * nmax + 0.5 * nmax * nmax) * (1.0 + 1e-9);
+  const double Dk = __builtin_sqrt(dk) * (1.0 + 1e-12);   // covers the rerank's own rounding
+  const double r = Dk + rq + rmax;
+  const double tn = 0.5 * (r * r - qn2) + alpha;
+  return tn + 1e-12 * (r * r + qn2 + 1.0);                  // FP64 rounding of this expression
+}
+
 __device__ void certify_query(const FinalArgs& a, int q, int k, int Kq, double dk) {
   const uint32_t t = a.thr[q];
   uint32_t need = 0;
   if (Kq >= k && t != ORD_POS_INF) {
-    const double qn2 = a.qn2[q], rq = a.qrho[q];
-    const double rmax = (double)__builtin_bit_cast(float, a.bnd[BND_RHO + a.bslot]);
-    const double nmax = (double)__builtin_bit_cast(float, a.bnd[BND_NRM + a.bslot]);
-    const double alpha = a.gamma * (__builtin_sqrt(qn2) * nmax + 0.5 * nmax * nmax) * (1.0 + 1e-9);
-    const double Dk = __builtin_sqrt(dk) * (1.0 + 1e-12);   // covers the rerank's own rounding
-    const double r = Dk + rq + rmax;
-    double tn = 0.5 * (r * r - qn2) + alpha;
-    tn += 1e-12 * (r * r + qn2 + 1.0);                         // FP64 rounding of this expression
+    const double tn = key_beyond(a, q, dk);
     const double tau = (double)ord2f(t);
     if (!(tau > tn)) {
       need = f2ord((float)next_up_f32(tn));
@@ -1790,101 +1816,139 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
   int cnt = a.carry_cnt[q];
   if (cnt > FW_MAX) cnt = FW_MAX;
   const int K = a.kc[q];
-  const int Kq = cnt;   // every carry entry: the shortlist and its certificate band
   const int k = a.kq[q];
   const int d = a.d;
   const uint64_t* src = a.carry + (long)q * a.carry_cap;
-  for (int b = 0; b < Kq; b += 256) {
-    uint32_t idx[4];
-    bool ok[4];
+  auto sort_first = [&](int count) {   // sorts dk / di [0, count) by (distance, row)
+    int P = 64;
+    while (P < count) P <<= 1;
+    wave_sync();
+    for (int i = count + lane; i < P; i += 64) { dk[i] = ~0ull; di[i] = ~0u; }
+    wave_bitonic_pair_r8(dk, di, P);   // 64 <= P <= FW_MAX
+  };
+  // The merges write the carry's K' smallest keys first and the certificate band after them.
+  // Phase 1 reranks the first min(K', cnt) entries; the k-th of their exact distances bounds the
+  // carry's k-th from above (it is the k-th of a subset), so a band entry whose key proves it
+  // farther than that bound cannot be among the k nearest, and phase 2 reranks only the others.
+  // (The pruning needs no order in the carry; the order only makes the bound tight.) One copy of
+  // the gather serves both phases: list[0, count) -> dk / di [dst, dst + count), list being the
+  // carry (phase 1) or this wave's LDS (phase 2: each slot is read, then written, by one lane).
+  const int Kc = K < cnt ? K : cnt;
+  const uint64_t* list = src;
+  int count = Kc, dst = 0, nr = Kc;
+  bool sorted = false;   // dk / di [0, nr) already in (distance, row) order
+  for (int phase = 0;; ++phase) {
+      for (int b = 0; b < count; b += 256) {
+        uint32_t idx[4];
+        bool ok[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int t = b + u * 64 + lane;
-      ok[u] = t < Kq;
-      idx[u] = ok[u] ? (uint32_t)(src[t] & 0xffffffffu) : 0u;   // row 0 stands in for idle lanes
-    }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    if (!a.master_f64) {
-      const float* qr = (const float*)a.Qm + (long)q * a.ldqm;
-      const float* r[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) r[u] = (const float*)a.Xm + (long)idx[u] * a.ldxm;
-      int j = 0;
-      if ((a.ldxm & 3) == 0 && (a.ldqm & 3) == 0) {
-        // 8 x 16 bytes per row per step: 32 independent loads in flight per lane (the gather is
-        // latency-bound, rows are scattered over the corpus)
-        constexpr int U = 8;
-        for (; j + 4 * U <= d; j += 4 * U) {
-          f32x4 qv[U], rv[U][4];
-#pragma unroll
-          for (int w = 0; w < U; ++w) {
-            qv[w] = *(const f32x4*)(qr + j + 4 * w);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) rv[w][u] = *(const f32x4*)(r[u] + j + 4 * w);
-          }
-#pragma unroll
-          for (int w = 0; w < U; ++w)
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-#pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const double t1 = (double)qv[w][e] - (double)rv[w][u][e];
-                acc[u] = __builtin_fma(t1, t1, acc[u]);
-              }
+        for (int u = 0; u < 4; ++u) {
+          const int t = b + u * 64 + lane;
+          ok[u] = t < count;
+          idx[u] = ok[u] ? (uint32_t)(list[t] & 0xffffffffu) : 0u;   // row 0 stands in for idle lanes
         }
-        for (; j + 4 <= d; j += 4) {
-          const f32x4 qv = *(const f32x4*)(qr + j);
-          f32x4 rv[4];
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!a.master_f64) {
+          const float* qr = (const float*)a.Qm + (long)q * a.ldqm;
+          const float* r[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) rv[u] = *(const f32x4*)(r[u] + j);
+          for (int u = 0; u < 4; ++u) r[u] = (const float*)a.Xm + (long)idx[u] * a.ldxm;
+          int j = 0;
+          if ((a.ldxm & 3) == 0 && (a.ldqm & 3) == 0) {
+            // 8 x 16 bytes per row per step: 32 independent loads in flight per lane (the gather
+            // is latency-bound, rows are scattered over the corpus)
+            constexpr int U = 8;
+            for (; j + 4 * U <= d; j += 4 * U) {
+              f32x4 qv[U], rv[U][4];
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
+              for (int w = 0; w < U; ++w) {
+                qv[w] = *(const f32x4*)(qr + j + 4 * w);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) rv[w][u] = *(const f32x4*)(r[u] + j + 4 * w);
+              }
+#pragma unroll
+              for (int w = 0; w < U; ++w)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+#pragma unroll
+                  for (int u = 0; u < 4; ++u) {
+                    const double t1 = (double)qv[w][e] - (double)rv[w][u][e];
+                    acc[u] = __builtin_fma(t1, t1, acc[u]);
+                  }
+            }
+            for (; j + 4 <= d; j += 4) {
+              const f32x4 qv = *(const f32x4*)(qr + j);
+              f32x4 rv[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) rv[u] = *(const f32x4*)(r[u] + j);
+#pragma unroll
+              for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  const double t1 = (double)qv[e] - (double)rv[u][e];
+                  acc[u] = __builtin_fma(t1, t1, acc[u]);
+                }
+            }
+          }
+          for (; j < d; ++j) {
+            const double qj = (double)qr[j];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-              const double t1 = (double)qv[e] - (double)rv[u][e];
+              const double t1 = qj - (double)r[u][j];
               acc[u] = __builtin_fma(t1, t1, acc[u]);
             }
+          }
+        } else {
+          const double* qr = (const double*)a.Qm + (long)q * a.ldqm;
+          const double* r[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) r[u] = (const double*)a.Xm + (long)idx[u] * a.ldxm;
+          for (int j = 0; j < d; ++j) {
+            const double qj = qr[j];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const double t1 = qj - r[u][j];
+              acc[u] = __builtin_fma(t1, t1, acc[u]);
+            }
+          }
         }
-      }
-      for (; j < d; ++j) {
-        const double qj = (double)qr[j];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const double t1 = qj - (double)r[u][j];
-          acc[u] = __builtin_fma(t1, t1, acc[u]);
+          const int t = b + u * 64 + lane;
+          if (ok[u]) { dk[dst + t] = __builtin_bit_cast(uint64_t, acc[u]); di[dst + t] = idx[u]; }
         }
       }
-    } else {
-      const double* qr = (const double*)a.Qm + (long)q * a.ldqm;
-      const double* r[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) r[u] = (const double*)a.Xm + (long)idx[u] * a.ldxm;
-      for (int j = 0; j < d; ++j) {
-        const double qj = qr[j];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double t1 = qj - r[u][j];
-          acc[u] = __builtin_fma(t1, t1, acc[u]);
-        }
-      }
+    if (phase == 1 || cnt <= Kc) break;
+    const bool prune = a.prune && Kc >= k;
+    uint32_t need = 0;
+    if (prune) {
+      sort_first(Kc);
+      sorted = true;
+      need = f2ord((float)next_up_f32(key_beyond(a, q, __builtin_bit_cast(double, dk[k - 1]))));
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int t = b + u * 64 + lane;
-      if (ok[u]) { dk[t] = __builtin_bit_cast(uint64_t, acc[u]); di[t] = idx[u]; }
+    for (int b = Kc; b < cnt; b += 64) {
+      const int t = b + lane;
+      const uint64_t e = t < cnt ? src[t] : ~0ull;
+      const bool keep = t < cnt && (!prune || (uint32_t)(e >> 32) < need);
+      const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+      const int pos = nr + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
+      if (keep) dk[pos] = e;
+      nr += __popcll(km);
     }
+    wave_sync();
+    if (nr == Kc) break;   // every band entry pruned: the first Kc are the answer, sorted
+    sorted = false;
+    list = dk + Kc;
+    count = nr - Kc;
+    dst = Kc;
   }
-  int P = 64;
-  while (P < Kq) P <<= 1;
-  wave_sync();
-  for (int i = Kq + lane; i < P; i += 64) { dk[i] = ~0ull; di[i] = ~0u; }
-  wave_bitonic_pair_r8(dk, di, P);   // 64 <= P <= FW_MAX
-  if (a.certify && lane == 0) certify_query(a, q, k, Kq, Kq >= k ? __builtin_bit_cast(double, dk[k - 1]) : 0.0);
+  if (!sorted) sort_first(nr);
+  if (a.certify && lane == 0) certify_query(a, q, k, nr, nr >= k ? __builtin_bit_cast(double, dk[k - 1]) : 0.0);
   const long o = a.out_off[q];
   for (int t = lane; t < k; t += 64) {
     double dist;
     int32_t id;
-    if (t < Kq) { dist = __builtin_bit_cast(double, dk[t]); id = (int32_t)di[t]; }
+    if (t < nr) { dist = __builtin_bit_cast(double, dk[t]); id = (int32_t)di[t]; }
     else { dist = __builtin_inf(); id = -1; }
     if (a.out_f64) ((double*)a.out_dist)[o + t] = dist;
     else ((float*)a.out_dist)[o + t] = (float)dist;
