@@ -21,6 +21,8 @@ struct knns_corpus_s {
   uint16_t* x16 = nullptr;
   double* x64 = nullptr;
   uint32_t* bnd = nullptr; // certificate maxima per operand copy (PackArgs::bnd), BND_SLOTS words
+  float* rrho = nullptr;   // per-row rounding radius ||x~ - x|| of the FP32 (slot 0) and BF16
+                           // (slot 1) operand rows, rounded up: [2][rows_pad] (PackArgs::rrho)
   uint8_t* x8 = nullptr;   // FP8 (E4M3) rows with their scale, rows_pad x ld8 bytes
   int dp8 = 0;             // FP8 row bytes (d rounded up to 64, 128 or 256)
   long ld8 = 0;            // dp8 + 16 (the augmentation unit)

@@ -496,6 +496,8 @@ struct Upload {
     pa.ld8 = c->ld8;
     pa.dp8 = c->dp8;
     pa.bnd = c->bnd;
+    pa.rrho = c->rrho ? c->rrho + r0 : nullptr;
+    pa.rrho_ld = c->rows_pad;
     if (launch_pack(pa, st) != hipSuccess || (c->x8 && launch_pack_fp8(pa, st) != hipSuccess))
       return fail(KNNS_ERR_DEVICE, "corpus pack launch failed");
     if (hipEventRecord(ready, st) != hipSuccess) return fail(KNNS_ERR_DEVICE, "hipEventRecord failed");
@@ -675,6 +677,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   fa.bslot = P.prec_kernel;
   fa.gamma = key_gamma(P);
   fa.prune = band.on;
+  fa.rrho = c->rrho && pk != PREC_FP8 ? c->rrho + (pk == PREC_BF16 ? c->rows_pad : 0) : nullptr;
   if (L.feed) KNNS_TRY(feed_rows(*L.feed, c->rows_pad, st));
   if (hook) (*hook)(3);
   KNNS_TRY(launch_final(fa, st));
@@ -1372,7 +1375,9 @@ knns_status corpus_alloc(knns_corpus* out, const void* data, int dtype, long m, 
   // whole scan stages (64 or 128 rows) never leave the padding, also for a search over a row
   // range (knns_search_rows) whose slices start at any row: 128 rows of slack past round_up(m, 128)
   c->rows_pad = round_up(m, 128) + 128;
-  c->ld32 = c->dp + 4;
+  // FP32 rows [dp dims | hi, lo, 0, 0] on a 64-byte stride: the exact rerank gathers a row's first
+  // d values from whole 64-byte sectors (8 per 128-dim row instead of 9 on a 528-byte stride)
+  c->ld32 = round_up(c->dp + 4, 16);
   c->ld16 = c->dp + 16;
   if (flags & KNNS_CORPUS_FP8) {
     c->dp8 = fp8_row_bytes(d);
@@ -1380,6 +1385,7 @@ knns_status corpus_alloc(knns_corpus* out, const void* data, int dtype, long m, 
   }
   auto cleanup = [&](knns_status st) {
     if (c->bnd) (void)hipFree(c->bnd);
+    if (c->rrho) (void)hipFree(c->rrho);
     if (c->x32) (void)hipFree(c->x32);
     if (c->x16) (void)hipFree(c->x16);
     if (c->x8) (void)hipFree(c->x8);
@@ -1400,6 +1406,8 @@ knns_status corpus_alloc(knns_corpus* out, const void* data, int dtype, long m, 
   // certificate maxima, cleared on the upload stream and raised by the pack kernels
   if (hipMalloc((void**)&c->bnd, BND_SLOTS * sizeof(uint32_t)) != hipSuccess)
     return cleanup(fail(KNNS_ERR_NOMEM, "corpus bounds"));
+  if (hipMalloc((void**)&c->rrho, 2 * (size_t)c->rows_pad * sizeof(float)) != hipSuccess)
+    return cleanup(fail(KNNS_ERR_NOMEM, "corpus row radii"));
   *out = c;
   return KNNS_OK;
 }
@@ -1434,6 +1442,7 @@ knns_status knns_corpus_destroy(knns_corpus c) {
   Engine& E = engine();
   std::lock_guard<std::recursive_mutex> lock(E.mu);
   if (c->bnd) (void)hipFree(c->bnd);
+  if (c->rrho) (void)hipFree(c->rrho);
   if (c->x32) (void)hipFree(c->x32);
   if (c->x16) (void)hipFree(c->x16);
   if (c->x8) (void)hipFree(c->x8);

@@ -128,6 +128,9 @@ struct FinalArgs {
   // carry's first min(K', cnt) entries exactly, then skips every further entry whose key proves it
   // farther than the k-th of those (the same bound as the certificate's)
   int prune;
+  const float* rrho;     // optional: per-row rounding radius of the search's operand rows (rounded
+                         // up); the pruning then bounds each band entry with its own row's radius
+                         // instead of the corpus maximum
 };
 
 struct PackArgs {
@@ -154,6 +157,9 @@ struct PackArgs {
   double* qn2;           // query rows: ||q~||^2 of the operand of precision qprec (one per row)
   double* qrho;          // query rows: ||q~ - q|| of that operand, rounded up
   int qprec;             // Prec of the search's operand (which copy qn2 / qrho describe)
+  float* rrho;           // corpus rows: ||x~ - x|| per row, rounded up, of the FP32 operand at
+                         // rrho[row] and of the BF16 operand at rrho[rrho_ld + row]
+  long rrho_ld;
 };
 constexpr int BND_RHO = 0, BND_NRM = 3, BND_SLOTS = 8;
 

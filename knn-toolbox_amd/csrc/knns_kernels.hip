@@ -226,9 +226,11 @@ __device__ __forceinline__ void pack_bounds(const PackArgs& a, long row, int slo
       a.qrho[row] = (double)sqrt_up(err2);
     }
   } else if (a.bnd && row < a.rows) {
+    const float rho = sqrt_up(err2);
+    if (a.rrho && slot <= PREC_BF16) a.rrho[slot * a.rrho_ld + row] = rho;
     // an atomic only when the row raises the running maximum (the maxima settle after a few rows;
     // a stale read only costs an unneeded atomic)
-    const uint32_t rb = __builtin_bit_cast(uint32_t, sqrt_up(err2)), nb = __builtin_bit_cast(uint32_t, sqrt_up(nrm2));
+    const uint32_t rb = __builtin_bit_cast(uint32_t, rho), nb = __builtin_bit_cast(uint32_t, sqrt_up(nrm2));
     if (rb > __hip_atomic_load(&a.bnd[BND_RHO + slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       atomicMax(&a.bnd[BND_RHO + slot], rb);
     if (nb > __hip_atomic_load(&a.bnd[BND_NRM + slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
@@ -1805,20 +1807,39 @@ __device__ void wave_bitonic_pair_r8(uint64_t* k, uint32_t* v, int P) {
   wave_sync();
 }
 
+constexpr int FW_COOP_D = 256;   // widest rows of the cooperative gather (query row kept in LDS)
+
+// COOP: FP32 master rows of 16-byte aligned stride, d a multiple of 16 up to FW_COOP_D (the
+// cooperative gather below); otherwise the per-lane gather. Two instantiations keep each gather's
+// registers apart.
+__host__ __device__ inline bool final_coop_ok(const FinalArgs& a) {
+  return !a.master_f64 && (a.ldxm & 3) == 0 && (a.ldqm & 3) == 0 && (a.d & 15) == 0 && a.d <= FW_COOP_D;
+}
+
+template <bool COOP>
 __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
   __shared__ __attribute__((aligned(16))) uint64_t dk_all[4][FW_MAX];
   __shared__ __attribute__((aligned(16))) uint32_t di_all[4][FW_MAX];
+  __shared__ f32x4 tile_all[4][COOP ? 64 * 4 : 1];            // 64 rows x 16 dims per wave
+  __shared__ f32x4 qs_all[4][COOP ? FW_COOP_D / 4 : 1];       // the query row
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = blockIdx.x * 4 + wv;
   if (q >= (a.n_dev ? *a.n_dev : a.n)) return;   // wave-uniform
   uint64_t* dk = dk_all[wv];
   uint32_t* di = di_all[wv];
+  f32x4* tile = tile_all[wv];
+  f32x4* qs = qs_all[wv];
   int cnt = a.carry_cnt[q];
   if (cnt > FW_MAX) cnt = FW_MAX;
   const int K = a.kc[q];
   const int k = a.kq[q];
   const int d = a.d;
   const uint64_t* src = a.carry + (long)q * a.carry_cap;
+  constexpr bool coop = COOP;
+  if (coop) {
+    const f32x4* qr = (const f32x4*)((const float*)a.Qm + (long)q * a.ldqm);
+    for (int i = lane; i < d / 4; i += 64) qs[i] = qr[i];
+  }
   auto sort_first = [&](int count) {   // sorts dk / di [0, count) by (distance, row)
     int P = 64;
     while (P < count) P <<= 1;
@@ -1837,8 +1858,69 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
   const uint64_t* list = src;
   int count = Kc, dst = 0, nr = Kc;
   bool sorted = false;   // dk / di [0, nr) already in (distance, row) order
+  // the rounding radii of the first 256 band entries' rows, loaded before phase 1's gather
+  const bool row_radii = a.prune && a.rrho && cnt > Kc;
+  float rxp[4] = {0.f, 0.f, 0.f, 0.f};
+  if (row_radii) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = Kc + 64 * u + lane;
+      if (t < cnt) rxp[u] = a.rrho[(uint32_t)(src[t] & 0xffffffffu)];
+    }
+  }
   for (int phase = 0;; ++phase) {
-      for (int b = 0; b < count; b += 256) {
+    // Cooperative gather: lanes 4j .. 4j+3 load the four 16-byte parts of one row's 64-byte chunk
+    // (16 dims), so a load instruction covers 16 rows and every sector is requested by one
+    // instruction only (per-lane row loads touch each 64-byte sector from four instructions, three
+    // of which wait for the first one's miss). Each chunk goes through this wave's LDS tile (parts
+    // swizzled by row so that the per-row reads are free of bank conflicts) to the lane that owns
+    // the row, whose FMA chain takes the dims in order: the same canonical sum as the per-lane form.
+    for (int b = 0; coop && b < count; b += 64) {
+      constexpr int FC = 8;                            // chunks in flight: 32 loads per lane
+      const int slot = lane >> 2, part = lane & 3;
+      const float* rp[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int i = b + 16 * t + slot;
+        const uint32_t row = i < count ? (uint32_t)(list[i] & 0xffffffffu) : 0u;   // row 0 for idle slots
+        rp[t] = (const float*)a.Xm + (long)row * a.ldxm + 4 * part;
+      }
+      const int mine = b + lane;
+      const bool okm = mine < count;
+      const uint32_t myrow = okm ? (uint32_t)(list[mine] & 0xffffffffu) : 0u;
+      double acc = 0.0;
+      for (int c0 = 0; c0 < d; c0 += 16 * FC) {
+        f32x4 v[FC][4];
+#pragma unroll
+        for (int c = 0; c < FC; ++c)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (c0 + 16 * c < d) v[c][t] = *(const f32x4*)(rp[t] + c0 + 16 * c);
+#pragma unroll
+        for (int c = 0; c < FC; ++c) {
+          if (c0 + 16 * c >= d) break;   // wave-uniform
+          wave_sync();                   // the previous chunk's reads of the tile are done
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int r = 16 * t + slot;
+            tile[r * 4 + (part ^ ((r >> 2) & 3))] = v[c][t];
+          }
+          wave_sync();
+#pragma unroll
+          for (int p = 0; p < 4; ++p) {
+            const f32x4 m = tile[lane * 4 + (p ^ ((lane >> 2) & 3))];
+            const f32x4 qv = qs[(c0 + 16 * c) / 4 + p];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const double t1 = (double)qv[e] - (double)m[e];
+              acc = __builtin_fma(t1, t1, acc);
+            }
+          }
+        }
+      }
+      if (okm) { dk[dst + mine] = __builtin_bit_cast(uint64_t, acc); di[dst + mine] = myrow; }
+    }
+      for (int b = 0; !coop && b < count; b += 256) {
         uint32_t idx[4];
         bool ok[4];
 #pragma unroll
@@ -1920,20 +2002,44 @@ __global__ __launch_bounds__(256) void final_wave_kernel(FinalArgs a) {
       }
     if (phase == 1 || cnt <= Kc) break;
     const bool prune = a.prune && Kc >= k;
-    uint32_t need = 0;
+    double dk1 = 0.0, rq = 0.0, qn2 = 0.0, alpha = 0.0, rmax = 0.0;
     if (prune) {
       sort_first(Kc);
       sorted = true;
-      need = f2ord((float)next_up_f32(key_beyond(a, q, __builtin_bit_cast(double, dk[k - 1]))));
+      // the certificate's bound (key_beyond) with the entry's own row radius when known
+      qn2 = a.qn2[q];
+      rq = a.qrho[q];
+      rmax = (double)__builtin_bit_cast(float, a.bnd[BND_RHO + a.bslot]);
+      const double nmax = (double)__builtin_bit_cast(float, a.bnd[BND_NRM + a.bslot]);
+      alpha = a.gamma * (__builtin_sqrt(qn2) * nmax + 0.5 * nmax * nmax) * (1.0 + 1e-9);
+      dk1 = __builtin_sqrt(__builtin_bit_cast(double, dk[k - 1])) * (1.0 + 1e-12);
     }
-    for (int b = Kc; b < cnt; b += 64) {
-      const int t = b + lane;
-      const uint64_t e = t < cnt ? src[t] : ~0ull;
-      const bool keep = t < cnt && (!prune || (uint32_t)(e >> 32) < need);
+    // a band entry stays unless its key proves its row farther than the k-th of phase 1
+    auto keep_entry = [&](uint64_t e, double rx) -> bool {
+      if (!prune) return true;
+      const double r = dk1 + rq + rx;
+      const double tn = 0.5 * (r * r - qn2) + alpha + 1e-12 * (r * r + qn2 + 1.0);
+      return (double)ord2f((uint32_t)(e >> 32)) <= tn;
+    };
+    auto emit = [&](int b, bool keep, uint64_t e) {
       const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
       const int pos = nr + __builtin_amdgcn_mbcnt_hi((uint32_t)(km >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)km, 0));
       if (keep) dk[pos] = e;
       nr += __popcll(km);
+    };
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = Kc + 64 * u;
+      if (b >= cnt) break;   // wave-uniform
+      const int t = b + lane;
+      const uint64_t e = t < cnt ? src[t] : ~0ull;
+      emit(b, t < cnt && keep_entry(e, row_radii ? (double)rxp[u] : rmax), e);
+    }
+    for (int b = Kc + 256; b < cnt; b += 64) {
+      const int t = b + lane;
+      const uint64_t e = t < cnt ? src[t] : ~0ull;
+      const double rx = row_radii && t < cnt ? (double)a.rrho[(uint32_t)(e & 0xffffffffu)] : rmax;
+      emit(b, t < cnt && keep_entry(e, rx), e);
     }
     wave_sync();
     if (nr == Kc) break;   // every band entry pruned: the first Kc are the answer, sorted
@@ -2129,10 +2235,13 @@ hipError_t launch_merge(const MergeArgs& a, hipStream_t stream) {
 hipError_t launch_final(const FinalArgs& a, hipStream_t stream) {
   if (a.n <= 0) return hipSuccess;
   if (a.carry_cap > FINAL_MAX) return hipErrorInvalidValue;
-  if (a.carry_cap <= FW_MAX)
-    hipLaunchKernelGGL(final_wave_kernel, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
-  else
+  if (a.carry_cap > FW_MAX) {
     hipLaunchKernelGGL(final_block_kernel, dim3(a.n), dim3(256), 0, stream, a);
+  } else if (final_coop_ok(a)) {
+    hipLaunchKernelGGL(final_wave_kernel<true>, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL(final_wave_kernel<false>, dim3((a.n + 3) / 4), dim3(256), 0, stream, a);
+  }
   return hipGetLastError();
 }
 
