@@ -42,7 +42,7 @@ import knns  # noqa: E402
 QBS = (2, 1, 3, 4)             # the built-in choice for n > 32 first
 WAVES = (5, 3, 4, 6, 8)        # 5 = built-in
 RATIOS = (0, 2, 3, 4)          # 0 = built-in schedule
-EXTRAS = (0, -32, 32, 64)
+EXTRAS = (0, -8, -16, -32, -48, 32)
 
 
 def mixture(n: int, d: int, seed: int, centers: np.ndarray, chunk: int = 1 << 20) -> np.ndarray:
