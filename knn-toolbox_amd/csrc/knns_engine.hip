@@ -238,10 +238,15 @@ Tuning tuning_for(int prec, int d, int k, long n, long m) {
     if (!path.empty() && load_tuning(path.c_str(), &v) == KNNS_OK) g_tune = std::move(v);
     g_err = keep;
   }
+  // FP64 searches run the FP32 scan and shortlist rule: without FP64 entries they take the FP32 ones
+  int use = prec;
+  if (prec == KNNS_PREC_FP64 &&
+      std::none_of(g_tune.begin(), g_tune.end(), [](const TuneEntry& e) { return e.prec == KNNS_PREC_FP64; }))
+    use = KNNS_PREC_FP32;
   Tuning best;
   double bd = 1e30;
   for (const TuneEntry& e : g_tune) {
-    if (e.prec != prec) continue;
+    if (e.prec != use) continue;
     const double dist = std::fabs(std::log2((double)d / e.d)) + std::fabs(std::log2((double)k / e.k)) +
                         std::fabs(std::log2((double)n / e.n)) + std::fabs(std::log2((double)m / e.m));
     if (dist < bd) { bd = dist; best = e.t; }

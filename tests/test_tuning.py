@@ -69,6 +69,23 @@ def test_nearest_entry_and_precision(knns_mod, tmp_path, restore_table):
     assert knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")["slices_max"] == default
 
 
+def test_fp64_searches_use_fp32_entries_without_their_own(knns_mod, tmp_path, restore_table):
+    """FP64 searches run the FP32 scan and shortlist rule: with no fp64 entry in the table they
+    take the nearest fp32 entry; an fp64 entry, when present, wins."""
+    L = knns_mod.native()
+    t = tmp_path / "f.tsv"
+    t.write_text("fp32 32 10 10000 1000000 5 0 2 -16\n")
+    assert L.knns_set_tuning_table(str(t).encode()) == 0
+    p = knns_mod.plan_describe(200_000, 32, 200_000, 8, "fp64")
+    assert p["qb"] == 2 and p["shortlist"] == 8          # k + 16 - 16
+    t.write_text("fp32 32 10 10000 1000000 5 0 2 -16\nfp64 32 10 10000 1000000 5 0 1 0\n")
+    assert L.knns_set_tuning_table(str(t).encode()) == 0
+    p = knns_mod.plan_describe(200_000, 32, 200_000, 8, "fp64")
+    assert p["qb"] == 1 and p["shortlist"] == 24
+    assert L.knns_set_tuning_table(None) == 0
+    assert knns_mod.plan_describe(200_000, 32, 200_000, 8, "fp64")["qb"] == 1
+
+
 @pytest.mark.parametrize("text,where", [("bf16 128 100\n", ":1:"), ("# c\nfp99 1 1 1 1 5 0 0 0\n", ":2: unknown precision"),
                                         ("bf16 128 100 10 100 0 0 0 0\n", "out of range"), ("bf16 128 100 10 100 5 1 0 0\n", "out of range"),
                                         ("bf16 128 100 10 100 5 0 7 0\n", "out of range"), ("bf16 128 100 10 0 5 0 0 0\n", "out of range"),
