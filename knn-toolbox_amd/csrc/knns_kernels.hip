@@ -70,6 +70,15 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, int lane) {
 }
 __device__ __forceinline__ int bits64(uint64_t x) { return x ? 64 - __clzll(x) : 0; }
 
+// The lane index through an empty asm. Rarely taken paths inside the scan loop (appends, in-scan
+// compaction) use it so that the values they derive from the lane index are computed there,
+// instead of being hoisted out of the loop into registers that are then held for the whole scan.
+__device__ __forceinline__ int lane_opaque() {
+  int l = (int)(threadIdx.x & 63);
+  asm volatile("" : "+v"(l));
+  return l;
+}
+
 __device__ __forceinline__ double next_up_f32(double x) {   // a float strictly above x
   const float f = (float)x;
   float g = (double)f > x ? f : __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, f) + (f >= 0.f ? 1u : -1u));
@@ -113,7 +122,7 @@ template <bool WAVE, class Each>
 __device__ __forceinline__ uint64_t group_kth_each(Each each, int N, int K, uint32_t* lds) {
   uint32_t* hist = lds;                          // 256 bins
   uint64_t* red = (uint64_t*)(lds + 256);        // 2 x (waves) u64, waves <= 4
-  const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int tid = WAVE ? lane_opaque() : (int)threadIdx.x;
   const int nt = WAVE ? 64 : (int)blockDim.x;
   const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   uint64_t mn = ~0ull, mx = 0;
@@ -203,7 +212,7 @@ __device__ __forceinline__ uint64_t group_kth_each(Each each, int N, int K, uint
 
 template <bool WAVE, class Get>
 __device__ __forceinline__ uint64_t group_kth(Get get, int N, int K, uint32_t* lds) {
-  const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int tid = WAVE ? lane_opaque() : (int)threadIdx.x;
   const int nt = WAVE ? 64 : (int)blockDim.x;
   return group_kth_each<WAVE>([&](auto f) { for (int i = tid; i < N; i += nt) f(get(i)); }, N, K, lds);
 }
@@ -429,7 +438,7 @@ __device__ __forceinline__ uint64_t pool_key(uint64_t e) {
 template <int QB>
 __device__ __forceinline__ void scan_compact(const ScanArgs& a, ScanState<QB>& st, int g, int sl,
                                           uint32_t* lds, int room) {
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_opaque();
   const int lim = a.sub_cap - room;
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
@@ -1025,6 +1034,279 @@ __global__ __launch_bounds__(256, QB >= 3 ? 1 : 2) void scan_swz_kernel(ScanArgs
     const int q = (g * QB + qb) * 32 + col;
     const int fill = a.dense ? (h == 0 && c1 > c0 ? (int)(c1 - c0) : 0) : st.cnt[qb];
     a.sub_cnt[((long)q * a.slices_max + sl) * 2 + h] = fill;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Pipelined BF16 / FP8 threshold scan
+// ------------------------------------------------------------------------------------------------
+// The stage layout, DMA, query operands and filter rule of scan_swz_kernel, with each 128-row stage
+// cut into four tiles of 32 rows (one row block x the wave's QB query blocks) that run as a
+// software pipeline, so that LDS latency and the filter hide under MFMA issue:
+//   * a tile's operand registers are refilled with the next tile's units right after their last
+//     MFMA: a whole tile of MFMAs separates a read from its use (one k-step in scan_swz_kernel,
+//     shorter than the LDS latency under load);
+//   * the next tile's norm step is issued in the middle of the current chain, so its result is
+//     ready when that tile starts; it is the C operand of every query block's first step;
+//   * a tile's filter runs during the next tile's chain: its max tree in parts between the k-steps'
+//     MFMAs (no branches), its appends after the chain. The last tile of a stage is filtered during
+//     the first tile of the next stage (the room check between stages still covers four tiles).
+// A scheduling barrier after each k-step keeps this order (the compiler otherwise sinks the reads
+// next to their use).
+// Rows past the slice end (a partial last stage) are computed, never appended: the max tree does
+// not mask them (a false hit only enters the append path), the append checks each row.
+// Dense rounds (no threshold yet) run scan_swz_kernel.
+template <int QB>
+struct TileHits {
+  float gm[QB][4];    // per query block: the lane's maximum over each group of 4 consecutive rows
+  uint64_t hit[QB];   // ballot over the wave: some lane of the query block holds a survivor
+};
+
+// Part `part` of NPARTS of a tile's max tree: an even share of its QB x 4 group maxima; the last
+// part also reduces them and takes the ballots. The parts are spread over the k-steps of the next
+// tile's MFMA chain (the loop is unrolled: the selection folds away).
+template <int QB, int NPARTS>
+__device__ __forceinline__ void tile_max_part(const ScanState<QB>& st, const f32x16 (&acc)[QB], TileHits<QB>& th,
+                                              int part) {
+  constexpr int NG = QB * 4;
+#pragma unroll
+  for (int x = 0; x < NG; ++x) {
+    if (x * NPARTS / NG != part) continue;
+    const int qb = x / 4, gi = x % 4;
+    th.gm[qb][gi] = fmaxf(fmaxf(acc[qb][4 * gi], acc[qb][4 * gi + 1]),
+                          fmaxf(acc[qb][4 * gi + 2], acc[qb][4 * gi + 3]));
+  }
+  if (part != NPARTS - 1) return;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const float qm = fmaxf(fmaxf(th.gm[qb][0], th.gm[qb][1]), fmaxf(th.gm[qb][2], th.gm[qb][3]));
+    th.hit[qb] = __builtin_amdgcn_ballot_w64(qm > st.phi[qb]);
+  }
+}
+
+// Appends of one tile: lane value v is row t0 + 8*(v/4) + (v%4) + 4*half; rows at or past `lim`
+// (relative to t0) lie beyond the slice and are skipped.
+template <int QB, bool FULL>
+__device__ __forceinline__ void tile_append(const ScanArgs& a, ScanState<QB>& st, const f32x16 (&acc)[QB],
+                                            const TileHits<QB>& th, uint32_t t0, int lim) {
+  const int h4 = (lane_opaque() >> 5) * 4;
+  const uint32_t tb = t0 + (uint32_t)h4;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    if (th.hit[qb] == 0) continue;
+    const float phi = st.phi[qb];
+    uint64_t* pool = a.sub + st.pbase(a, qb);
+    int cnt = st.cnt[qb];
+#pragma unroll
+    for (int gi = 0; gi < 4; ++gi) {
+      if (__builtin_amdgcn_ballot_w64(th.gm[qb][gi] > phi) == 0) continue;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float D = acc[qb][4 * gi + t];
+        if ((FULL || 8 * gi + t + h4 < lim) && D > phi) {
+          pool[cnt] = ((uint64_t)__builtin_bit_cast(uint32_t, D) << 32) | (tb + (uint32_t)(8 * gi + t));
+          ++cnt;
+        }
+      }
+    }
+    st.cnt[qb] = cnt;
+  }
+}
+
+template <int QB>
+__device__ __forceinline__ void tile_filter(const ScanArgs& a, ScanState<QB>& st, const f32x16 (&acc)[QB],
+                                            const TileHits<QB>& th, uint32_t t0, int c1) {
+  const int lim = c1 - (int)t0;
+  if (lim >= 32) tile_append<QB, true>(a, st, acc, th, t0, lim);
+  else if (lim > 0) tile_append<QB, false>(a, st, acc, th, t0, lim);
+}
+
+template <int QB, int SU, bool F8>
+__global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
+  using G = SwzGeom<SU>;
+  constexpr int ESZ = F8 ? 1 : 2;                // bytes per element
+  constexpr int J = F8 ? SU / 4 : SU / 2;        // MFMA k-steps per tile
+  constexpr int TILES = G::SR / 32;
+  constexpr uint32_t TILE_MAIN = 32 * SU * 16;   // LDS bytes of a tile's main units
+  __shared__ __attribute__((aligned(1024))) uint8_t smem[2 * G::MAIN_BYTES + 2 * G::AUG_BYTES];
+  __shared__ __attribute__((aligned(16))) uint32_t kth_lds[4][256 + 32];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar addressing
+  const int col = lane & 31, h = lane >> 5;
+  const int wgrow = blockIdx.x, sl = blockIdx.y;   // workgroup row (4 query groups), slice
+  if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
+  const int g = wgrow * 4 + wv;
+  const int c0 = (int)(a.row_begin + (long)sl * a.slice_rows);
+  int c1 = c0 + (int)a.slice_rows;
+  if (c1 > (int)a.row_end) c1 = (int)a.row_end;
+
+  ScanState<QB> st;
+  st.q0 = g * QB * 32;
+  st.sl = sl;
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int q = (g * QB + qb) * 32 + col;
+    st.phi[qb] = -ord2f(a.thr[q]);
+    st.cnt[qb] = 0;
+  }
+  const int nst = c0 < c1 ? (c1 - c0 + G::SR - 1) / G::SR : 0;
+  const uint8_t* X = (const uint8_t*)a.X;
+  const uint32_t ldb = (uint32_t)a.ldx * ESZ;
+
+  // corpus DMA: the loads, descriptor and offsets of scan_swz_kernel
+  const int swl = (4 * wv + (lane >> 4)) % SU;
+  const uint32_t dma_off = (uint32_t)(lane / SU) * ldb + (uint32_t)((lane % SU) ^ swl) * 16u;
+  const uint32_t aug_off = (uint32_t)lane * ldb + (uint32_t)a.dp * ESZ;
+  typedef __attribute__((address_space(3))) uint8_t lds_u8;
+  const int kstride = (int)(4 * G::RPL) * (int)ldb;
+  const int voff = (int)(dma_off + (uint32_t)(wv * G::RPL) * ldb);
+  const int aoff = (int)(aug_off + (uint32_t)(wv * 64) * ldb);
+  auto issue = [&](int t0, int buf) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(X + (size_t)t0 * ldb), (short)0, (int)(G::SR * ldb), 0x00020000);
+    int soff = 0;
+#pragma unroll
+    for (int k = 0; k < G::PER_WAVE; ++k) {
+      lds_u8* l = (lds_u8*)(smem + buf * G::MAIN_BYTES + wv * 1024 + k * 4096);
+      asm volatile("" : "+s"(l));
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (void __attribute__((address_space(3)))*)l, 16, voff, soff,
+                                               0, 0);
+      soff += kstride;
+    }
+    if (wv < G::SR / 64) {
+      lds_u8* la = (lds_u8*)(smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES + wv * 1024);
+      asm volatile("" : "+s"(la));
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (void __attribute__((address_space(3)))*)la, 16, aoff, 0, 0, 0);
+    }
+  };
+
+  // per-wave query operand (B), resident for the whole slice
+  typedef typename std::conditional<F8, i32x8, bf16x8>::type OT;
+  OT bq[QB][J];
+  int sq[QB];
+  {
+    const uint8_t* Q = (const uint8_t*)a.Qp;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      const uint8_t* qrow = Q + (size_t)((g * QB + qb) * 32 + col) * a.ldq * ESZ;
+      const uint8_t* qp = qrow + h * (SU / 2) * 16;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        if constexpr (F8) {
+          const u32x4 lo = *(const u32x4*)(qp + 32 * j), hi = *(const u32x4*)(qp + 32 * j + 16);
+          bq[qb][j] = (i32x8){(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3],
+                              (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+        } else {
+          bq[qb][j] = __builtin_bit_cast(bf16x8, *(const u32x4*)(qp + 16 * j));
+        }
+      }
+      sq[qb] = F8 ? (int)(*(const uint32_t*)(qrow + a.dp + 8) & 0xFFu) : 0;
+    }
+  }
+  bf16x8 baug;
+  {
+    const uint32_t one = 0x3f80u, w01 = h ? 0u : (one | (one << 16)), w2 = h ? 0u : one;
+    baug = __builtin_bit_cast(bf16x8, (u32x4){w01, w2, 0u, 0u});
+  }
+  // operand read offsets: a tile's rows start at a multiple of 32, so the swizzle of its row col
+  // depends on col only, as in scan_swz_kernel
+  const int swc = (col / (16 / SU)) % SU;
+  const uint32_t A0 = (uint32_t)col * SU * 16u + (uint32_t)(((h * (SU / 2)) ^ swc) * 16);
+  const uint32_t AG = (uint32_t)col * 16u;
+  constexpr uint32_t STEP = F8 ? 32u : 16u;
+  auto rd = [&](const uint8_t* p, uint32_t off) -> u32x4 { return *(const u32x4*)(p + off); };
+  auto rdop = [&](const uint8_t* p, uint32_t off) -> OT {
+    if constexpr (F8) {
+      const u32x4 lo = rd(p, off), hi = rd(p, off ^ 16u);
+      return (i32x8){(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3],
+                     (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
+    } else {
+      return __builtin_bit_cast(bf16x8, rd(p, off));
+    }
+  };
+  auto mma = [&](const OT& x, int qb, int j, const f32x16& c, int sx) -> f32x16 {
+    if constexpr (F8) {
+      // formats 0, 0 = E4M3 x E4M3; scale bytes in bits 0..7 (op_sel 0)
+      return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x, bq[qb][j], c, 0, 0, 0, sx, 0, sq[qb]);
+    } else {
+      return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, bq[qb][j], c, 0, 0, 0);
+    }
+  };
+  auto norm_step = [&](const u32x4& gu) -> f32x16 {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, gu), baug, (f32x16){0}, 0, 0, 0);
+  };
+
+  OT xa[J];                 // operands of the tile being multiplied
+  u32x4 xg;                 // its augmentation unit (FP8: byte 8 = the row's scale exponent)
+  f32x16 nrm;               // norm step of the tile (read by its first k-step only)
+  f32x16 acc[2][QB];        // accumulators (ping-pong by tile parity)
+  TileHits<QB> th;
+  // the tile before the first one: nothing to filter
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) acc[1][qb] = (f32x16){0};
+  uint32_t prev_t0 = (uint32_t)c0;
+  int prev_c1 = c0;         // lim = prev_c1 - prev_t0 <= 0: no appends
+
+  // the previous tile's max tree runs in NPARTS parts at k-steps 1 .. J-1 (step 0 when J = 1): its
+  // last MFMAs have finished by then
+  constexpr int NPARTS = J > 1 ? J - 1 : 1;
+  if (nst > 0) issue(c0, 0);
+  for (int s = 0; s < nst; ++s) {
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's DMA for stage s has landed
+    __syncthreads();                 // ... every wave's; and nobody still reads the other buffer
+    const int buf = s & 1;
+    const int t0s = c0 + s * G::SR;
+    if (s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
+    const uint8_t* mb = smem + buf * G::MAIN_BYTES;
+    const uint8_t* ab = smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES;
+    xg = rd(ab, AG);
+#pragma unroll
+    for (int j = 0; j < J; ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
+    nrm = norm_step(xg);
+    int sx = F8 ? (int)(xg[2] & 0xFFu) : 0;
+#pragma unroll
+    for (int rb = 0; rb < TILES; ++rb) {
+      const int cur = rb & 1, pv = cur ^ 1;
+      const bool more = rb + 1 < TILES;
+      const uint8_t* mn = mb + (rb + 1) * TILE_MAIN;   // the next tile's main units (if more)
+      int sx_next = 0;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        if (more && j == 0) xg = rd(ab + (rb + 1) * 32 * 16, AG);
+#pragma unroll
+        for (int qb = QB - 1; qb >= 0; --qb) acc[cur][qb] = mma(xa[j], qb, j, j == 0 ? nrm : acc[cur][qb], sx);
+        if (more) xa[j] = rdop(mn, A0 ^ (STEP * (uint32_t)j));
+        if (more && j == J / 2) {
+          nrm = norm_step(xg);
+          if constexpr (F8) sx_next = (int)(xg[2] & 0xFFu);
+        }
+        if (J == 1 || j > 0) tile_max_part<QB, NPARTS>(st, acc[pv], th, J == 1 ? 0 : j - 1);
+        // keep this order: reads a whole tile ahead of their use, the max tree between MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // the previous tile's appends (the last tile of the previous stage when rb = 0)
+      if (rb > 0) tile_filter<QB>(a, st, acc[pv], th, (uint32_t)(t0s + 32 * (rb - 1)), c1);
+      else tile_filter<QB>(a, st, acc[pv], th, prev_t0, prev_c1);
+      if (more) sx = sx_next;
+    }
+    prev_t0 = (uint32_t)(t0s + 32 * (TILES - 1));
+    prev_c1 = c1;
+    // room until the next check: this stage's last tile and the next stage's first three, at most
+    // 4 x 16 appends per lane and query block
+    bool trig = false;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 64;
+    if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv], 64);
+  }
+  // the last tile
+#pragma unroll
+  for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS>(st, acc[1], th, p);
+  tile_filter<QB>(a, st, acc[1], th, prev_t0, prev_c1);
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb) {
+    const int q = (g * QB + qb) * 32 + col;
+    a.sub_cnt[((long)q * a.slices_max + sl) * 2 + h] = st.cnt[qb];
   }
 }
 
@@ -2189,6 +2471,20 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
   const int sr = scan_stage_rows(prec, a.dp, dc);
   if (a.row_begin < 0 || (a.slice_rows % sr)) return hipErrorInvalidValue;
   if (a.row_end > 0x7fffff80L) return hipErrorInvalidValue;   // int32 row indices
+  static const bool pipe = !(std::getenv("KNNS_SCAN_PIPE") && std::getenv("KNNS_SCAN_PIPE")[0] == '0');
+  if (pipe && !a.dense && (qb == 1 || qb == 2) && (prec == PREC_FP8 || (prec == PREC_BF16 && a.dp == dc && dc != 96))) {
+    if (prec == PREC_FP8 && a.dp != dc) return hipErrorInvalidValue;
+    const int su = prec == PREC_FP8 ? dc / 16 : dc / 8;
+    const bool f8 = prec == PREC_FP8;
+#define KNNS_PIPE_CASE(B, S, F) \
+    if (qb == B && su == S && f8 == F) return scan_grid(scan_pipe_kernel<B, S, F>, a, groups, stream);
+    KNNS_PIPE_CASE(1, 4, false) KNNS_PIPE_CASE(1, 8, false) KNNS_PIPE_CASE(1, 16, false)
+    KNNS_PIPE_CASE(2, 4, false) KNNS_PIPE_CASE(2, 8, false) KNNS_PIPE_CASE(2, 16, false)
+    KNNS_PIPE_CASE(1, 4, true) KNNS_PIPE_CASE(1, 8, true) KNNS_PIPE_CASE(1, 16, true)
+    KNNS_PIPE_CASE(2, 4, true) KNNS_PIPE_CASE(2, 8, true) KNNS_PIPE_CASE(2, 16, true)
+#undef KNNS_PIPE_CASE
+    return hipErrorInvalidValue;
+  }
   if (prec == PREC_FP8) {
     // dc = dp = FP8 row bytes (64, 128, 256); ScanArgs strides are in bytes
     if (a.dp != dc) return hipErrorInvalidValue;
