@@ -182,7 +182,7 @@ hipError_t launch_gather_rows(void* dst, const void* src, const int* idx, long n
 hipError_t launch_fill_i32(int* p, int v, long n, hipStream_t stream);
 bool scan_supported(int prec, int qb, int dc);
 bool scan_qb_ok(int prec, int dp, int dc, int qb);
-int scan_wgs_per_cu(int qb);
+int scan_wgs_per_cu(int prec, int dp, int dc, int qb);
 int scan_stage_rows(int prec, int dp, int dc);
 
 }  // namespace knns

@@ -2450,14 +2450,24 @@ bool scan_supported(int prec, int qb, int dc) {
   return dc == 32 || dc == 64 || dc == 96 || dc == 128;
 }
 
-// Query blocks of 32 per wave that the scan is built for: 1 and 2 everywhere; 3 and 4 for the
-// single-chunk BF16 scan of 64- and 128-dim rows (one resident workgroup per CU: their queries
-// and accumulators take 400+ registers per lane).
+// Query blocks of 32 per wave that the scan is built for: 1 and 2 everywhere; 3 for the FP8 scan
+// of 64- and 128-byte rows; 3 and 4 for the single-chunk BF16 scan of 64- and 128-dim rows.
+// The pipelined threshold scan fits 3 blocks in two resident workgroups per CU for rows of up to
+// 8 units (FP8 rows of 64 and 128 bytes, BF16 rows of 32 and 64 dims); wider rows with 3 or 4
+// blocks (scan_swz_kernel, 400+ registers per lane) leave one workgroup per CU.
+static int scan_units(int prec, int dc) { return prec == PREC_FP8 ? dc / 16 : dc / 8; }
+static bool scan_pipe_ok(int prec, int dp, int dc, int qb) {
+  if (!(prec == PREC_FP8 || (prec == PREC_BF16 && dp == dc && dc != 96))) return false;
+  return qb == 1 || qb == 2 || (qb == 3 && scan_units(prec, dc) <= 8);
+}
 bool scan_qb_ok(int prec, int dp, int dc, int qb) {
   if (qb == 1 || qb == 2) return true;
+  if (qb == 3 && prec == PREC_FP8) return dc == 64 || dc == 128;
   return (qb == 3 || qb == 4) && prec == PREC_BF16 && dp == dc && (dc == 64 || dc == 128);
 }
-int scan_wgs_per_cu(int qb) { return qb >= 3 ? 1 : 2; }
+int scan_wgs_per_cu(int prec, int dp, int dc, int qb) {
+  return qb <= 2 || scan_pipe_ok(prec, dp, dc, qb) ? 2 : 1;
+}
 
 // Row granularity of the stages (64 or 128 rows): slice sizes are multiples of it. A slice may
 // start at any row; the corpus carries 128 rows past round_up(m, 128), so the last stage of a slice
@@ -2472,9 +2482,9 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
   if (a.row_begin < 0 || (a.slice_rows % sr)) return hipErrorInvalidValue;
   if (a.row_end > 0x7fffff80L) return hipErrorInvalidValue;   // int32 row indices
   static const bool pipe = !(std::getenv("KNNS_SCAN_PIPE") && std::getenv("KNNS_SCAN_PIPE")[0] == '0');
-  if (pipe && !a.dense && (qb == 1 || qb == 2) && (prec == PREC_FP8 || (prec == PREC_BF16 && a.dp == dc && dc != 96))) {
+  if (pipe && !a.dense && scan_pipe_ok(prec, a.dp, dc, qb)) {
     if (prec == PREC_FP8 && a.dp != dc) return hipErrorInvalidValue;
-    const int su = prec == PREC_FP8 ? dc / 16 : dc / 8;
+    const int su = scan_units(prec, dc);
     const bool f8 = prec == PREC_FP8;
 #define KNNS_PIPE_CASE(B, S, F) \
     if (qb == B && su == S && f8 == F) return scan_grid(scan_pipe_kernel<B, S, F>, a, groups, stream);
@@ -2482,6 +2492,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
     KNNS_PIPE_CASE(2, 4, false) KNNS_PIPE_CASE(2, 8, false) KNNS_PIPE_CASE(2, 16, false)
     KNNS_PIPE_CASE(1, 4, true) KNNS_PIPE_CASE(1, 8, true) KNNS_PIPE_CASE(1, 16, true)
     KNNS_PIPE_CASE(2, 4, true) KNNS_PIPE_CASE(2, 8, true) KNNS_PIPE_CASE(2, 16, true)
+    KNNS_PIPE_CASE(3, 4, true) KNNS_PIPE_CASE(3, 8, true) KNNS_PIPE_CASE(3, 4, false) KNNS_PIPE_CASE(3, 8, false)
 #undef KNNS_PIPE_CASE
     return hipErrorInvalidValue;
   }
@@ -2492,6 +2503,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
     if (qb == B && dc == D) return scan_grid(scan_swz_kernel<B, D / 16, true>, a, groups, stream);
     KNNS_F8_CASE(1, 64) KNNS_F8_CASE(1, 128) KNNS_F8_CASE(1, 256)
     KNNS_F8_CASE(2, 64) KNNS_F8_CASE(2, 128) KNNS_F8_CASE(2, 256)
+    KNNS_F8_CASE(3, 64) KNNS_F8_CASE(3, 128)
 #undef KNNS_F8_CASE
     return hipErrorInvalidValue;
   }

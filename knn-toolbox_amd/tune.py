@@ -69,6 +69,8 @@ def qb_ok(prec: str, d: int, qb: int) -> bool:
     """Tile shapes the scan is built for (others fall back to the built-in rule)."""
     if qb in (1, 2):
         return True
+    if prec == "fp8":
+        return qb == 3 and d <= 128
     return prec == "bf16" and d in (64, 128)
 
 

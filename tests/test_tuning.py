@@ -156,6 +156,35 @@ def test_tile_shapes_give_identical_results(gpu, knns_mod, oracle_mod, tmp_path,
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec,d", [("fp8", 96), ("fp8", 64), ("bf16", 64)])
+def test_three_block_tiles_give_identical_results(gpu, knns_mod, oracle_mod, tmp_path, restore_table, prec, d):
+    """Three query blocks per wave (the pipelined scan of FP8 rows up to 128 bytes and of BF16 rows
+    up to 64 dims) compute the same keys as one and two: identical answers, and for BF16 the
+    oracle's."""
+    L = knns_mod.native()
+    R = oracle_mod.mixture(150000, d, 128, 0.1, center_seed=81, point_seed=82)
+    Q = oracle_mod.mixture(900, d, 128, 0.1, center_seed=81, point_seed=83)
+    k = 20
+    Do, Io = oracle_mod.exact_topk(Q[:48], R, k)
+    with knns_mod.Corpus(R, bf16=True, fp8=prec == "fp8") as c:
+        ref = None
+        for qb in (1, 2, 3):
+            t = tmp_path / f"t{qb}.tsv"
+            t.write_text(f"{prec} {d} {k} 900 150000 5 0 {qb} 0\n")
+            assert L.knns_set_tuning_table(str(t).encode()) == 0
+            print(f"\n[case] {prec} d={d} tile qb={qb}", flush=True)
+            D, I = c.search(Q, k, precision=prec)
+            assert knns_mod.last_stats()["qb"] == qb
+            if prec == "bf16":
+                assert np.array_equal(I[:48], Io) and np.array_equal(D[:48], Do.astype(np.float32))
+            else:
+                assert oracle_mod.recall_at_k(I[:48], Io) > 0.9
+            if ref is None:
+                ref = (D, I)
+            assert np.array_equal(I, ref[1]) and np.array_equal(D, ref[0])
+
+
+@pytest.mark.gpu
 def test_server_shortlist_follows_each_k(gpu, knns_mod, oracle_mod, tmp_path, restore_table):
     """A server's shortlist for a request of k is the one a plain search of that k would use: the
     table's offset for max_k (here -32 at k = 100) is not applied to k = 1 requests, whose own
