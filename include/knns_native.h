@@ -153,6 +153,11 @@ knns_status knns_search_once(const void* data, int dtype, long m, int d, const v
 /* Planner view: the launch configuration knns_search would use (no GPU work). */
 knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns_stats* out);
 
+/* Rank of the entry that sets a certified search's threshold after the fraction `seen` of its
+ * rows, for a shortlist of `shortlist` rows (the shortlist itself at seen = 1). No counterpart in
+ * the reference; exposed so that the bound it is built on can be checked. */
+int knns_predicted_rank(int shortlist, double seen);
+
 const char* knns_last_error(void);
 knns_status knns_last_stats(knns_stats* out);
 int knns_device_count(void);

@@ -654,6 +654,9 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     ma.self_ids = (const int*)B.self;
     ma.n_dev = L.n_dev;
     ma.band = band;
+    ma.seen = r + 1 == P.rounds.size()
+                  ? 1.0
+                  : (double)(R.end - P.rounds[0].begin) / (double)(P.rounds.back().end - P.rounds[0].begin);
     KNNS_TRY(launch_merge(ma, st));
     if (hook) (*hook)(2);
     cur ^= 1;
@@ -678,6 +681,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
   fa.n = (int)n;
   fa.n_dev = L.n_dev;
   fa.certify = L.certify && B.qn2 ? 1 : 0;
+  fa.rows = std::max<long>(0, (P.rounds.empty() ? 0 : P.rounds.back().end - P.rounds[0].begin) - (L.excl ? 1 : 0));
   fa.thr = (uint32_t*)B.thr;
   fa.flags = (int*)B.flags;
   fa.qn2 = (const double*)B.qn2;
@@ -1553,6 +1557,8 @@ knns_status knns_plan_describe(long m, int d, long n, int k, int precision, knns
   out->shortlist = cl;
   return KNNS_OK;
 }
+
+int knns_predicted_rank(int shortlist, double seen) { return shortlist < 1 ? 0 : predicted_rank(shortlist, seen); }
 
 knns_status knns_set_tuning_table(const char* path) {
   g_err.clear();

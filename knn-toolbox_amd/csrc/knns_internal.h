@@ -71,6 +71,19 @@ struct ScanArgs {
   uint32_t* thr_w;       // thr, writable (the same array)
 };
 
+// Rank of the entry that sets a merge's threshold before the last round, for a shortlist of K
+// rows after the fraction `seen` of the search's rows. When the rows are in no particular order,
+// the number of the final K nearest rows inside that prefix is Binomial(K, seen); the rank is an
+// upper quantile of it, mu + 5.2 sqrt(mu) + 5.4 with mu = K seen, which bounds the binomial tail
+// by 1e-7 for every K up to 2048 (checked against scipy in the tests), capped at K. The last
+// merge (seen = 1) keeps the K smallest.
+__host__ __device__ inline int predicted_rank(int K, double seen) {
+  if (seen >= 1.0) return K;
+  const double mu = (double)K * (seen > 0.0 ? seen : 0.0);
+  const int r = (int)ceil(mu + 5.2 * sqrt(mu) + 5.4);
+  return r < K ? r : K;
+}
+
 struct MergeArgs {
   const uint64_t* carry_in;  // [q][carry_cap]
   const int* carry_in_cnt;
@@ -91,6 +104,7 @@ struct MergeArgs {
   const int* self_ids;   // optional: corpus row excluded for each query (all-kNN), -1 = none
   const int* n_dev;      // optional: real query count on the device (overrides n)
   BandArgs band;         // keep every entry below f(K'-th key) (up to carry_cap), threshold f(K'-th)
+  double seen;           // fraction of the search's rows scanned after this round (1 after the last)
 };
 
 struct FinalArgs {
@@ -119,6 +133,8 @@ struct FinalArgs {
   int certify;
   uint32_t* thr;
   int* flags;
+  long rows;             // candidate rows of a query (the searched rows, less the excluded self row):
+                         // a shortlist shorter than k is complete only when it holds them all
   const double* qn2;     // ||q~||^2 per query (operand as rounded for the scan)
   const double* qrho;    // ||q~ - q|| per query, rounded up
   const uint32_t* bnd;   // corpus maxima (PackArgs::bnd); slot bslot

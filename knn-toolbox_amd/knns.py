@@ -184,6 +184,15 @@ def last_stats() -> dict:
     return s.as_dict()
 
 
+def predicted_rank(shortlist: int, seen: float) -> int:
+    """knns_predicted_rank: the rank that sets a certified search's threshold after the fraction
+    `seen` of its rows (the shortlist itself at seen = 1)."""
+    L = native()
+    L.knns_predicted_rank.argtypes = [ctypes.c_int, ctypes.c_double]
+    L.knns_predicted_rank.restype = ctypes.c_int
+    return int(L.knns_predicted_rank(int(shortlist), float(seen)))
+
+
 def plan_describe(m: int, d: int, n: int, k: int, precision: str = "fp32") -> dict:
     s = Stats()
     st = native().knns_plan_describe(m, d, n, k, PREC[precision], ctypes.byref(s))

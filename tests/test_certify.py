@@ -95,6 +95,26 @@ def _shell(seed=5, d=128, n_shell=2000, n_far=30000, n_q=6):
     return R, Q
 
 
+def test_predicted_rank_bounds_the_binomial_tail(knns_mod):
+    """Before the last round a certified search ranks by knns_predicted_rank(K', f): the number of a
+    query's K' nearest rows inside a prefix holding the fraction f of rows in no particular order is
+    Binomial(K', f), and the rank is an upper quantile of it (tail at most 1e-7), never above K',
+    non-decreasing in f, and K' itself once every row is seen."""
+    from scipy.stats import binom
+    fs = [1e-6, 1e-4, 1e-3, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.3, 0.5, 0.7, 0.9, 0.99]
+    for K in list(range(1, 301)) + list(range(308, 2049, 20)):
+        prev = 0
+        for f in fs:
+            r = knns_mod.predicted_rank(K, f)
+            assert prev <= r <= K
+            prev = r
+            if r < K:
+                assert binom.sf(r - 1, K, f) <= 1e-7, (K, f, r)
+        assert knns_mod.predicted_rank(K, 1.0) == K
+    # the headline's early merges (K' = 118 after 1024, 4096, 16384 of 10^6 rows) rank far below K'
+    assert all(knns_mod.predicted_rank(118, f) <= 20 for f in (0.001024, 0.004096, 0.016384))
+
+
 @pytest.fixture(scope="module")
 def shell(oracle_mod):
     R, Q = _shell()
@@ -135,6 +155,29 @@ def test_band_overflow_is_researched(gpu, knns_mod, oracle_mod, prec):
         st = knns_mod.last_stats()
     assert np.array_equal(Ig, I) and np.array_equal(Dg, D.astype(np.float32))
     assert st["uncertified"] > 0 and st["researched"] == st["uncertified"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_prefix_holding_the_neighbours_is_researched(gpu, knns_mod, oracle_mod, prec):
+    """Before the last round the merges rank by the predicted share of the K' nearest rows in the
+    prefix seen so far (true for rows in no particular order). A corpus whose first rows are one
+    query's 400 nearest breaks that prediction for the query: its shortlist comes out short, the
+    certificate fails, the re-search runs and the answer is the oracle's; the other queries are
+    certified in one pass."""
+    R = oracle_mod.mixture(200000, 64, 64, 0.1, center_seed=91, point_seed=92)
+    Q = oracle_mod.mixture(64, 64, 64, 0.1, center_seed=91, point_seed=93)
+    d0 = ((R.astype(np.float64) - Q[0].astype(np.float64)) ** 2).sum(1)
+    near = np.argsort(d0, kind="stable")[:400]
+    rest = np.setdiff1d(np.arange(R.shape[0]), near)
+    R = np.ascontiguousarray(R[np.concatenate([near, rest])])
+    D, I = oracle_mod.exact_topk(Q, R, 100)
+    _case("neighbours_first", prec=prec)
+    with knns_mod.Corpus(R, bf16=True) as c:
+        Dg, Ig = c.search(Q, 100, precision=prec)
+        st = knns_mod.last_stats()
+    assert np.array_equal(Ig, I) and np.array_equal(Dg, D.astype(np.float32))
+    assert 1 <= st["researched"] < 16 and st["researched"] == st["uncertified"]
 
 
 @pytest.mark.gpu
