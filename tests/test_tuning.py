@@ -40,14 +40,14 @@ def test_table_drives_slices_and_ratio(knns_mod, tmp_path, restore_table):
         assert L.knns_set_tuning_table(str(t).encode()) == 0
         slices[w] = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")["slices_max"]
     assert slices[3] < slices[5] < slices[8]
-    # ratio: fixed x2 growth gives more rounds than the default schedule, x8 fewer
+    # ratio: fixed x2 growth gives more rounds than the default schedule, x16 fewer
     rounds = {}
-    for r in (0, 2, 8):
+    for r in (0, 2, 16):
         t = tmp_path / f"r{r}.tsv"
         t.write_text(f"bf16 128 100 10000 1000000 5 {r} 0 0\n")
         assert L.knns_set_tuning_table(str(t).encode()) == 0
         rounds[r] = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")["rounds"]
-    assert rounds[2] > rounds[0] > rounds[8]
+    assert rounds[2] > rounds[0] > rounds[16]
 
 
 def test_nearest_entry_and_precision(knns_mod, tmp_path, restore_table):
