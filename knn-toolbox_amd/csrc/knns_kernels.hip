@@ -1787,16 +1787,19 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
       }
     }
   } else {
-    // per-entry source map, then all loads of the lane in flight at once
+    // per-entry source map of the sub-pool entries (the carry's entries, slots [0, off[1]), need
+    // none: one lane would otherwise write all of them), then all loads of the lane in flight
 #pragma unroll
     for (int u = 0; u < 5; ++u) {
       const int sidx = lane * P + u;
-      if (u < P && sidx < nsrc) {
+      if (u < P && sidx > 0 && sidx < nsrc) {
         const int o = off[sidx];
         for (int j = 0; j < c[u]; ++j) map[o + j] = (uint16_t)sidx;
       }
     }
     wave_sync();
+    const int ncarry = off[1];
+    const int used = (N + 63) >> 6;   // slots holding entries (wave-uniform)
     // branch-free: every lane computes a valid address for each slot (idle slots read the first
     // carry entry), so all U loads are in flight together and one wait covers them; the key
     // conversion follows in a second pass
@@ -1806,8 +1809,9 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
     // past N read stale values that the selects discard), so they too are in flight together
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int m = (int)map[lane + 64 * u];
-      src[u] = lane + 64 * u < N ? m : -1;
+      const int i = lane + 64 * u;
+      const int m = (int)map[i];
+      src[u] = i < N ? (i < ncarry ? 0 : m) : -1;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) jj[u] = off[src[u] < 0 ? 0 : src[u]];
@@ -1818,7 +1822,7 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
     for (int u = 0; u < U; ++u) {
       const int s = src[u] < 0 ? 0 : src[u];
       const uint64_t* p = s == 0 ? cin + jj[u] : sub + (long)(s - 1) * a.sub_cap + jj[u];
-      v[u] = *p;
+      v[u] = u < used ? *p : ~0ull;   // slots past the union: no load
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
