@@ -1241,14 +1241,20 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, gu), baug, (f32x16){0}, 0, 0, 0);
   };
 
+  // OVL: a tile is filtered during the next tile's chain (two accumulator sets). Three query blocks
+  // of 16-unit rows leave no registers for a second set: each tile is filtered right after its own
+  // chain, while the next tile's norm step runs (and the SIMD's other wave keeps the MFMAs busy).
+  constexpr bool OVL = !(QB >= 3 && SU == 16);
   OT xa[J];                 // operands of the tile being multiplied
   u32x4 xg;                 // its augmentation unit (FP8: byte 8 = the row's scale exponent)
   f32x16 nrm;               // norm step of the tile (read by its first k-step only)
-  f32x16 acc[2][QB];        // accumulators (ping-pong by tile parity)
+  f32x16 acc[OVL ? 2 : 1][QB];   // accumulators (ping-pong by tile parity when OVL)
   TileHits<QB> th;
   // the tile before the first one: nothing to filter
+  if constexpr (OVL) {
 #pragma unroll
-  for (int qb = 0; qb < QB; ++qb) acc[1][qb] = (f32x16){0};
+    for (int qb = 0; qb < QB; ++qb) acc[1][qb] = (f32x16){0};
+  }
   uint32_t prev_t0 = (uint32_t)c0;
   int prev_c1 = c0;         // lim = prev_c1 - prev_t0 <= 0: no appends
 
@@ -1266,12 +1272,12 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     const uint8_t* ab = smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES;
     xg = rd(ab, AG);
 #pragma unroll
-    for (int j = 0; j < J; ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
+    for (int j = 0; j < (OVL ? J : 1); ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
     nrm = norm_step(xg);
     int sx = F8 ? (int)(xg[2] & 0xFFu) : 0;
 #pragma unroll
     for (int rb = 0; rb < TILES; ++rb) {
-      const int cur = rb & 1, pv = cur ^ 1;
+      const int cur = OVL ? rb & 1 : 0, pv = OVL ? cur ^ 1 : 0;
       const bool more = rb + 1 < TILES;
       const uint8_t* mn = mb + (rb + 1) * TILE_MAIN;   // the next tile's main units (if more)
       int sx_next = 0;
@@ -1280,18 +1286,30 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
         if (more && j == 0) xg = rd(ab + (rb + 1) * 32 * 16, AG);
 #pragma unroll
         for (int qb = QB - 1; qb >= 0; --qb) acc[cur][qb] = mma(xa[j], qb, j, j == 0 ? nrm : acc[cur][qb], sx);
-        if (more) xa[j] = rdop(mn, A0 ^ (STEP * (uint32_t)j));
+        if constexpr (OVL) {
+          if (more) xa[j] = rdop(mn, A0 ^ (STEP * (uint32_t)j));
+        } else {
+          // one k-step ahead (QB MFMAs cover the read): two operand sets live instead of J
+          if (j + 1 < J) xa[j + 1] = rdop(mb + rb * TILE_MAIN, A0 ^ (STEP * (uint32_t)(j + 1)));
+          else if (more) xa[0] = rdop(mn, A0);
+        }
         if (more && j == J / 2) {
           nrm = norm_step(xg);
           if constexpr (F8) sx_next = (int)(xg[2] & 0xFFu);
         }
-        if (J == 1 || j > 0) tile_max_part<QB, NPARTS>(st, acc[pv], th, J == 1 ? 0 : j - 1);
+        if (OVL && (J == 1 || j > 0)) tile_max_part<QB, NPARTS>(st, acc[pv], th, J == 1 ? 0 : j - 1);
         // keep this order: reads a whole tile ahead of their use, the max tree between MFMAs
         __builtin_amdgcn_sched_barrier(0);
       }
-      // the previous tile's appends (the last tile of the previous stage when rb = 0)
-      if (rb > 0) tile_filter<QB>(a, st, acc[pv], th, (uint32_t)(t0s + 32 * (rb - 1)), c1);
-      else tile_filter<QB>(a, st, acc[pv], th, prev_t0, prev_c1);
+      if constexpr (OVL) {
+        // the previous tile's appends (the last tile of the previous stage when rb = 0)
+        if (rb > 0) tile_filter<QB>(a, st, acc[pv], th, (uint32_t)(t0s + 32 * (rb - 1)), c1);
+        else tile_filter<QB>(a, st, acc[pv], th, prev_t0, prev_c1);
+      } else {
+#pragma unroll
+        for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS>(st, acc[cur], th, p);
+        tile_filter<QB>(a, st, acc[cur], th, (uint32_t)(t0s + 32 * rb), c1);
+      }
       if (more) sx = sx_next;
     }
     prev_t0 = (uint32_t)(t0s + 32 * (TILES - 1));
@@ -1303,10 +1321,11 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     for (int qb = 0; qb < QB; ++qb) trig |= st.cnt[qb] > a.sub_cap - 64;
     if (__builtin_amdgcn_ballot_w64(trig)) scan_compact<QB>(a, st, g, sl, kth_lds[wv], 64);
   }
-  // the last tile
+  if constexpr (OVL) {   // the last tile
 #pragma unroll
-  for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS>(st, acc[1], th, p);
-  tile_filter<QB>(a, st, acc[1], th, prev_t0, prev_c1);
+    for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS>(st, acc[1], th, p);
+    tile_filter<QB>(a, st, acc[1], th, prev_t0, prev_c1);
+  }
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
     const int q = (g * QB + qb) * 32 + col;
@@ -2482,7 +2501,7 @@ bool scan_supported(int prec, int qb, int dc) {
 static int scan_units(int prec, int dc) { return prec == PREC_FP8 ? dc / 16 : dc / 8; }
 static bool scan_pipe_ok(int prec, int dp, int dc, int qb) {
   if (!(prec == PREC_FP8 || (prec == PREC_BF16 && dp == dc && dc != 96))) return false;
-  return qb == 1 || qb == 2 || (qb == 3 && scan_units(prec, dc) <= 8);
+  return qb == 1 || qb == 2 || (qb == 3 && (scan_units(prec, dc) <= 8 || prec == PREC_BF16));
 }
 bool scan_qb_ok(int prec, int dp, int dc, int qb) {
   if (qb == 1 || qb == 2) return true;
@@ -2517,6 +2536,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
     KNNS_PIPE_CASE(1, 4, true) KNNS_PIPE_CASE(1, 8, true) KNNS_PIPE_CASE(1, 16, true)
     KNNS_PIPE_CASE(2, 4, true) KNNS_PIPE_CASE(2, 8, true) KNNS_PIPE_CASE(2, 16, true)
     KNNS_PIPE_CASE(3, 4, true) KNNS_PIPE_CASE(3, 8, true) KNNS_PIPE_CASE(3, 4, false) KNNS_PIPE_CASE(3, 8, false)
+    KNNS_PIPE_CASE(3, 16, false)
 #undef KNNS_PIPE_CASE
     return hipErrorInvalidValue;
   }

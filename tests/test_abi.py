@@ -85,8 +85,8 @@ def test_native_params_layout(knns_mod):
 
 def test_planner_shapes(knns_mod, builtin_plans):
     s = knns_mod.plan_describe(1_000_000, 128, 10_000, 100, "bf16")
-    assert s["qb"] == 2 and s["dc"] == 128 and s["dp"] == 128
-    assert s["groups"] == 160  # ceil(10000 / 64) rounded up to whole workgroups of 4 groups
+    assert s["qb"] == 3 and s["dc"] == 128 and s["dp"] == 128
+    assert s["groups"] == 108  # ceil(10000 / 96) rounded up to whole workgroups of 4 groups
     assert s["rounds"] >= 3 and 1 <= s["slices_max"] <= 128
     s = knns_mod.plan_describe(16384, 43, 1024, 16, "fp32")
     assert s["qb"] == 1 and s["dc"] == 64 and s["dp"] == 64 and s["groups"] == 32
