@@ -295,11 +295,11 @@ Plan make_plan(long m, int d, long n, int kmax, int cmax, int prec, int cus, int
   dims_for(d, &P.dp, &P.dc);
   P.prec_kernel = prec == KNNS_PREC_BF16 ? PREC_BF16 : (prec == KNNS_PREC_FP8 ? PREC_FP8 : PREC_F32);
   if (P.prec_kernel == PREC_FP8) P.dc = fp8_row_bytes(d);
-  // query blocks per scan wave: 2 for the low-precision scans; 3 with enough queries to fill the
-  // wider tiles for FP8 rows of up to 128 bytes (10M x 96, 10k queries: -8% scan time against 2)
-  // and for 128-dim BF16 rows (1M x 128, 10k queries: -2%)
+  // query blocks per scan wave: 2 for the low-precision scans; with enough queries to fill the
+  // wider tiles, 3 or 4 for FP8 rows of up to 128 bytes (10M x 96, 10k queries: scan 11.85 ms at
+  // 2 blocks, 10.92 at 3, 9.96 at 4) and 3 for 128-dim BF16 rows (1M x 128, 10k queries: -2%)
   P.qb = (P.prec_kernel != PREC_F32 && n > 32) ? 2 : 1;
-  if (P.prec_kernel == PREC_FP8 && P.dc <= 128 && n >= 2048) P.qb = 3;
+  if (P.prec_kernel == PREC_FP8 && P.dc <= 128 && n >= 2048) P.qb = n >= 4096 ? 4 : 3;
   if (P.prec_kernel == PREC_BF16 && P.dp == 128 && P.dc == 128 && n >= 4096) P.qb = 3;
   if (qb_req > 0 && scan_qb_ok(P.prec_kernel, P.dp, P.dc, qb_req)) P.qb = qb_req;
   const int qw = 32 * P.qb;

@@ -1242,9 +1242,10 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
   };
 
   // OVL: a tile is filtered during the next tile's chain (two accumulator sets). Three query blocks
-  // of 16-unit rows leave no registers for a second set: each tile is filtered right after its own
-  // chain, while the next tile's norm step runs (and the SIMD's other wave keeps the MFMAs busy).
-  constexpr bool OVL = !(QB >= 3 && SU == 16);
+  // of 16-unit rows, and four blocks of 8-unit rows, leave no registers for a second set: each tile
+  // is filtered right after its own chain, while the next tile's norm step runs (and the SIMD's
+  // other wave keeps the MFMAs busy). Four blocks of 16-unit rows do not fit in 256 registers.
+  constexpr bool OVL = QB <= 2 || (QB == 3 && SU <= 8);
   OT xa[J];                 // operands of the tile being multiplied
   u32x4 xg;                 // its augmentation unit (FP8: byte 8 = the row's scale exponent)
   f32x16 nrm;               // norm step of the tile (read by its first k-step only)
@@ -2493,23 +2494,24 @@ bool scan_supported(int prec, int qb, int dc) {
   return dc == 32 || dc == 64 || dc == 96 || dc == 128;
 }
 
-// Query blocks of 32 per wave that the scan is built for: 1 and 2 everywhere; 3 for the FP8 scan
-// of 64- and 128-byte rows; 3 and 4 for the single-chunk BF16 scan of 64- and 128-dim rows.
+// Query blocks of 32 per wave that the scan is built for: 1 and 2 everywhere; 3 and 4 for the FP8
+// scan of 64- and 128-byte rows; 3 and 4 for the single-chunk BF16 scan of 64- and 128-dim rows.
 // The pipelined threshold scan fits 3 blocks in two resident workgroups per CU for rows of up to
 // 8 units (FP8 rows of 64 and 128 bytes, BF16 rows of 32 and 64 dims); wider rows with 3 or 4
 // blocks (scan_swz_kernel, 400+ registers per lane) leave one workgroup per CU.
 static int scan_units(int prec, int dc) { return prec == PREC_FP8 ? dc / 16 : dc / 8; }
 static bool scan_pipe_ok(int prec, int dp, int dc, int qb) {
   if (!(prec == PREC_FP8 || (prec == PREC_BF16 && dp == dc && dc != 96))) return false;
-  return qb == 1 || qb == 2 || (qb == 3 && (scan_units(prec, dc) <= 8 || prec == PREC_BF16));
+  return qb == 1 || qb == 2 || (qb == 3 && (scan_units(prec, dc) <= 8 || prec == PREC_BF16)) ||
+         (qb == 4 && scan_units(prec, dc) <= 8 && (prec == PREC_FP8 || dc == 64));
 }
 bool scan_qb_ok(int prec, int dp, int dc, int qb) {
   if (qb == 1 || qb == 2) return true;
-  if (qb == 3 && prec == PREC_FP8) return dc == 64 || dc == 128;
+  if ((qb == 3 || qb == 4) && prec == PREC_FP8) return dc == 64 || dc == 128;
   return (qb == 3 || qb == 4) && prec == PREC_BF16 && dp == dc && (dc == 64 || dc == 128);
 }
 int scan_wgs_per_cu(int prec, int dp, int dc, int qb) {
-  return qb <= 2 || (qb == 3 && scan_pipe_ok(prec, dp, dc, qb)) ? 2 : 1;
+  return qb <= 2 || scan_pipe_ok(prec, dp, dc, qb) ? 2 : 1;
 }
 
 // Row granularity of the stages (64 or 128 rows): slice sizes are multiples of it. A slice may
@@ -2536,7 +2538,8 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
     KNNS_PIPE_CASE(1, 4, true) KNNS_PIPE_CASE(1, 8, true) KNNS_PIPE_CASE(1, 16, true)
     KNNS_PIPE_CASE(2, 4, true) KNNS_PIPE_CASE(2, 8, true) KNNS_PIPE_CASE(2, 16, true)
     KNNS_PIPE_CASE(3, 4, true) KNNS_PIPE_CASE(3, 8, true) KNNS_PIPE_CASE(3, 4, false) KNNS_PIPE_CASE(3, 8, false)
-    KNNS_PIPE_CASE(3, 16, false)
+    KNNS_PIPE_CASE(3, 16, false) KNNS_PIPE_CASE(4, 4, true) KNNS_PIPE_CASE(4, 8, true)
+    KNNS_PIPE_CASE(4, 8, false)
 #undef KNNS_PIPE_CASE
     return hipErrorInvalidValue;
   }
@@ -2547,7 +2550,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
     if (qb == B && dc == D) return scan_grid(scan_swz_kernel<B, D / 16, true>, a, groups, stream);
     KNNS_F8_CASE(1, 64) KNNS_F8_CASE(1, 128) KNNS_F8_CASE(1, 256)
     KNNS_F8_CASE(2, 64) KNNS_F8_CASE(2, 128) KNNS_F8_CASE(2, 256)
-    KNNS_F8_CASE(3, 64) KNNS_F8_CASE(3, 128)
+    KNNS_F8_CASE(3, 64) KNNS_F8_CASE(3, 128) KNNS_F8_CASE(4, 64) KNNS_F8_CASE(4, 128)
 #undef KNNS_F8_CASE
     return hipErrorInvalidValue;
   }

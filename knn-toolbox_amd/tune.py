@@ -8,8 +8,8 @@ Knobs (the role of src/knnsplan.cu:43-57, which timed 80 full runs of two algori
 stream counts; here every knob shapes the one fused kernel sequence):
   qb        query blocks of 32 per scan wave -- the tile shape: 32 qb queries x 32 corpus rows
             per MFMA tile, 4 query groups per workgroup sharing each corpus stage; 1 and 2 (two
-            resident workgroups per CU) everywhere, 3 for FP8 rows up to 128 bytes and BF16 rows
-            of 64 dims (two per CU), 3 and 4 (one per CU) for BF16 at d = 128 and 4 at d = 64;
+            resident workgroups per CU) everywhere, 3 and 4 for FP8 rows up to 128 bytes and BF16
+            rows of 64 dims and 3 for BF16 at d = 128 (two per CU), 4 (one per CU) at d = 128;
   min_waves slices per query group: the smallest count that fills that many waves of resident
             scan workgroups (the grid's second dimension);
   ratio     corpus growth between threshold rounds (0 = x4 while the prefix is below m/16, then x2);
@@ -71,7 +71,7 @@ def qb_ok(prec: str, d: int, qb: int) -> bool:
     if qb in (1, 2):
         return True
     if prec == "fp8":
-        return qb == 3 and d <= 128
+        return qb in (3, 4) and d <= 128
     return prec == "bf16" and d in (64, 128)
 
 

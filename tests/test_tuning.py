@@ -157,10 +157,10 @@ def test_tile_shapes_give_identical_results(gpu, knns_mod, oracle_mod, tmp_path,
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec,d", [("fp8", 96), ("fp8", 64), ("bf16", 64)])
-def test_three_block_tiles_give_identical_results(gpu, knns_mod, oracle_mod, tmp_path, restore_table, prec, d):
-    """Three query blocks per wave (the pipelined scan of FP8 rows up to 128 bytes and of BF16 rows
-    up to 64 dims) compute the same keys as one and two: identical answers, and for BF16 the
-    oracle's."""
+def test_wide_tiles_give_identical_results(gpu, knns_mod, oracle_mod, tmp_path, restore_table, prec, d):
+    """Three and four query blocks per wave (the pipelined scan of FP8 rows up to 128 bytes and of
+    64-dim BF16 rows; four blocks with one accumulator set) compute the same keys as one and two:
+    identical answers, and for BF16 the oracle's."""
     L = knns_mod.native()
     R = oracle_mod.mixture(150000, d, 128, 0.1, center_seed=81, point_seed=82)
     Q = oracle_mod.mixture(900, d, 128, 0.1, center_seed=81, point_seed=83)
@@ -168,7 +168,7 @@ def test_three_block_tiles_give_identical_results(gpu, knns_mod, oracle_mod, tmp
     Do, Io = oracle_mod.exact_topk(Q[:48], R, k)
     with knns_mod.Corpus(R, bf16=True, fp8=prec == "fp8") as c:
         ref = None
-        for qb in (1, 2, 3):
+        for qb in (1, 2, 3, 4):
             t = tmp_path / f"t{qb}.tsv"
             t.write_text(f"{prec} {d} {k} 900 150000 5 0 {qb} 0\n")
             assert L.knns_set_tuning_table(str(t).encode()) == 0
