@@ -1274,7 +1274,9 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     xg = rd(ab, AG);
 #pragma unroll
     for (int j = 0; j < (OVL ? J : 1); ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
-    nrm = norm_step(xg);
+    // the norm step; without OVL it goes straight into block 0's accumulator (no register of its own)
+    if constexpr (OVL) nrm = norm_step(xg);
+    else acc[0][0] = norm_step(xg);
     int sx = F8 ? (int)(xg[2] & 0xFFu) : 0;
 #pragma unroll
     for (int rb = 0; rb < TILES; ++rb) {
@@ -1286,7 +1288,10 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
       for (int j = 0; j < J; ++j) {
         if (more && j == 0) xg = rd(ab + (rb + 1) * 32 * 16, AG);
 #pragma unroll
-        for (int qb = QB - 1; qb >= 0; --qb) acc[cur][qb] = mma(xa[j], qb, j, j == 0 ? nrm : acc[cur][qb], sx);
+        // descending: without OVL, at j = 0 the blocks qb > 0 read block 0's norm before block 0
+        // adds to it
+        for (int qb = QB - 1; qb >= 0; --qb)
+          acc[cur][qb] = mma(xa[j], qb, j, j == 0 ? (OVL ? nrm : acc[cur][0]) : acc[cur][qb], sx);
         if constexpr (OVL) {
           if (more) xa[j] = rdop(mn, A0 ^ (STEP * (uint32_t)j));
         } else {
@@ -1295,7 +1300,7 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
           else if (more) xa[0] = rdop(mn, A0);
         }
         if (more && j == J / 2) {
-          nrm = norm_step(xg);
+          if constexpr (OVL) nrm = norm_step(xg);
           if constexpr (F8) sx_next = (int)(xg[2] & 0xFFu);
         }
         if (OVL && (J == 1 || j > 0)) tile_max_part<QB, NPARTS>(st, acc[pv], th, J == 1 ? 0 : j - 1);
@@ -1310,6 +1315,7 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
 #pragma unroll
         for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS>(st, acc[cur], th, p);
         tile_filter<QB>(a, st, acc[cur], th, (uint32_t)(t0s + 32 * rb), c1);
+        if (more) acc[0][0] = norm_step(xg);   // the next tile's norm, once this tile is filtered
       }
       if (more) sx = sx_next;
     }
