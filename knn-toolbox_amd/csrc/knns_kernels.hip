@@ -2532,8 +2532,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
   const int sr = scan_stage_rows(prec, a.dp, dc);
   if (a.row_begin < 0 || (a.slice_rows % sr)) return hipErrorInvalidValue;
   if (a.row_end > 0x7fffff80L) return hipErrorInvalidValue;   // int32 row indices
-  static const bool pipe = !(std::getenv("KNNS_SCAN_PIPE") && std::getenv("KNNS_SCAN_PIPE")[0] == '0');
-  if (pipe && !a.dense && scan_pipe_ok(prec, a.dp, dc, qb)) {
+  if (!a.dense && scan_pipe_ok(prec, a.dp, dc, qb)) {
     if (prec == PREC_FP8 && a.dp != dc) return hipErrorInvalidValue;
     const int su = scan_units(prec, dc);
     const bool f8 = prec == PREC_FP8;
