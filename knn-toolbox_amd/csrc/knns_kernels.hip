@@ -1247,6 +1247,8 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
   // other wave keeps the MFMAs busy). Four blocks of 16-unit rows do not fit in 256 registers.
   constexpr bool OVL = QB <= 2 || (QB == 3 && SU <= 8);
   OT xa[J];                 // operands of the tile being multiplied
+  // without OVL: operands read PD k-steps ahead of their MFMAs (crossing into the next tile)
+  constexpr int PD = OVL ? 0 : (J >= 2 ? 2 : 1);
   u32x4 xg;                 // its augmentation unit (FP8: byte 8 = the row's scale exponent)
   f32x16 nrm;               // norm step of the tile (read by its first k-step only)
   f32x16 acc[OVL ? 2 : 1][QB];   // accumulators (ping-pong by tile parity when OVL)
@@ -1273,7 +1275,7 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     const uint8_t* ab = smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES;
     xg = rd(ab, AG);
 #pragma unroll
-    for (int j = 0; j < (OVL ? J : 1); ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
+    for (int j = 0; j < (OVL ? J : PD); ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
     // the norm step; without OVL it goes straight into block 0's accumulator (no register of its own)
     if constexpr (OVL) nrm = norm_step(xg);
     else acc[0][0] = norm_step(xg);
@@ -1295,9 +1297,9 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
         if constexpr (OVL) {
           if (more) xa[j] = rdop(mn, A0 ^ (STEP * (uint32_t)j));
         } else {
-          // one k-step ahead (QB MFMAs cover the read): two operand sets live instead of J
-          if (j + 1 < J) xa[j + 1] = rdop(mb + rb * TILE_MAIN, A0 ^ (STEP * (uint32_t)(j + 1)));
-          else if (more) xa[0] = rdop(mn, A0);
+          // PD k-steps ahead (2 QB MFMAs cover the read): PD + 1 operand sets live instead of J
+          if (j + PD < J) xa[j + PD] = rdop(mb + rb * TILE_MAIN, A0 ^ (STEP * (uint32_t)(j + PD)));
+          else if (more) xa[j + PD - J] = rdop(mn, A0 ^ (STEP * (uint32_t)(j + PD - J)));
         }
         if (more && j == J / 2) {
           if constexpr (OVL) nrm = norm_step(xg);
