@@ -1273,12 +1273,13 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     __syncthreads();                 // ... every wave's; and nobody still reads the other buffer
     const int buf = s & 1;
     const int t0s = c0 + s * G::SR;
-    if (s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
     const uint8_t* mb = smem + buf * G::MAIN_BYTES;
     const uint8_t* ab = smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES;
     xg = rd(ab, AG);
 #pragma unroll
     for (int j = 0; j < (OVL ? J : PD); ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
+    // the next stage's DMA after this stage's first reads: their latency covers its issue
+    if (s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
     // the norm step; without OVL it goes straight into block 0's accumulator (no register of its own)
     if constexpr (OVL) nrm = norm_step(xg);
     else acc[0][0] = norm_step(xg);
