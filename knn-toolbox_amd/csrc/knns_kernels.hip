@@ -1278,8 +1278,10 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     xg = rd(ab, AG);
 #pragma unroll
     for (int j = 0; j < (OVL ? J : PD); ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
-    // the next stage's DMA after this stage's first reads: their latency covers its issue
-    if (s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
+    // the next stage's DMA: BF16 behind the stage's first MFMAs (they cover its issue), FP8 right
+    // after the first reads (measured: each the faster for its precision)
+    constexpr bool DMA_LATE = !F8;
+    if (!DMA_LATE && s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
     // the norm step; without OVL it goes straight into block 0's accumulator (no register of its own)
     if constexpr (OVL) nrm = norm_step(xg);
     else acc[0][0] = norm_step(xg);
@@ -1305,6 +1307,7 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
           if (j + PD < J) xa[j + PD] = rdop(mb + rb * TILE_MAIN, A0 ^ (STEP * (uint32_t)(j + PD)));
           else if (more) xa[j + PD - J] = rdop(mn, A0 ^ (STEP * (uint32_t)(j + PD - J)));
         }
+        if (DMA_LATE && rb == 0 && j == 0 && s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
         if (more && j == J / 2) {
           if constexpr (OVL) nrm = norm_step(xg);
           if constexpr (F8) sx_next = (int)(xg[2] & 0xFFu);
