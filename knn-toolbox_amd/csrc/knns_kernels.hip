@@ -1282,6 +1282,13 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     // after the first reads (measured: each the faster for its precision)
     constexpr bool DMA_LATE = !F8;
     if (!DMA_LATE && s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
+    if constexpr (!OVL) {   // the previous stage's last tile, while this stage's first reads land
+      if (s > 0) {
+#pragma unroll
+        for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS, true>(st, acc[0], th, p);
+        tile_filter<QB>(a, st, acc[0], th, prev_t0, prev_c1);
+      }
+    }
     // the norm step; without OVL it goes straight into block 0's accumulator (no register of its own)
     if constexpr (OVL) nrm = norm_step(xg);
     else acc[0][0] = norm_step(xg);
@@ -1320,11 +1327,13 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
         // the previous tile's appends (the last tile of the previous stage when rb = 0)
         if (rb > 0) tile_filter<QB>(a, st, acc[pv], th, (uint32_t)(t0s + 32 * (rb - 1)), c1);
         else tile_filter<QB>(a, st, acc[pv], th, prev_t0, prev_c1);
-      } else {
+      } else if (more) {
+        // (the stage's last tile is filtered after the next stage's barrier: its append stores
+        // then have a whole stage to complete before the wait for the next DMA)
 #pragma unroll
         for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS, true>(st, acc[cur], th, p);
         tile_filter<QB>(a, st, acc[cur], th, (uint32_t)(t0s + 32 * rb), c1);
-        if (more) acc[0][0] = norm_step(xg);   // the next tile's norm, once this tile is filtered
+        acc[0][0] = norm_step(xg);   // the next tile's norm, once this tile is filtered
       }
       if (more) sx = sx_next;
     }
@@ -1341,6 +1350,10 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
 #pragma unroll
     for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS>(st, acc[1], th, p);
     tile_filter<QB>(a, st, acc[1], th, prev_t0, prev_c1);
+  } else if (nst > 0) {
+#pragma unroll
+    for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS, true>(st, acc[0], th, p);
+    tile_filter<QB>(a, st, acc[0], th, prev_t0, prev_c1);
   }
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
