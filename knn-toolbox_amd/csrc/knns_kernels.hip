@@ -88,19 +88,17 @@ __device__ __forceinline__ double next_up_f32(double x) {   // a float strictly 
 
 // Threshold of the certificate band (BandArgs) for a K'-th key of order image t: the order image
 // of a float strictly above f(tau) = tau + D(tau) rho + rho^2 / 2 + 2 alpha; never below t.
-__device__ __forceinline__ uint32_t band_thr_of(const BandArgs& b, uint32_t t, double qn2, double qrho) {
+__device__ __forceinline__ uint32_t band_thr(const BandArgs& b, int q, uint32_t t) {
   if (t >= ORD_POS_INF) return t;
   const double tau = (double)ord2f(t);
+  const double qn2 = b.qn2[q];
   const double rmax = (double)__builtin_bit_cast(float, b.bnd[BND_RHO + b.bslot]);
   const double nmax = (double)__builtin_bit_cast(float, b.bnd[BND_NRM + b.bslot]);
   const double alpha = b.gamma * (__builtin_sqrt(qn2) * nmax + 0.5 * nmax * nmax);
-  const double rho = qrho + rmax;
+  const double rho = b.qrho[q] + rmax;
   const double D = __builtin_sqrt(__builtin_fmax(0.0, 2.0 * tau + qn2));
   const uint32_t r = f2ord((float)next_up_f32(tau + D * rho + 0.5 * rho * rho + 2.0 * alpha));
   return r > t ? r : t;
-}
-__device__ __forceinline__ uint32_t band_thr(const BandArgs& b, int q, uint32_t t) {
-  return t >= ORD_POS_INF ? t : band_thr_of(b, t, b.qn2[q], b.qrho[q]);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1067,15 +1065,14 @@ struct TileHits {
 // Part `part` of NPARTS of a tile's max tree: an even share of its QB x 4 group maxima; the last
 // part also reduces them and takes the ballots. The parts are spread over the k-steps of the next
 // tile's MFMA chain (the loop is unrolled: the selection folds away).
-template <int QB, int NPARTS, bool LAST_FIRST = false>
+template <int QB, int NPARTS>
 __device__ __forceinline__ void tile_max_part(const ScanState<QB>& st, const f32x16 (&acc)[QB], TileHits<QB>& th,
                                               int part) {
   constexpr int NG = QB * 4;
 #pragma unroll
   for (int x = 0; x < NG; ++x) {
     if (x * NPARTS / NG != part) continue;
-    // LAST_FIRST: block QB-1 first (its chain ends before block 0's, whose MFMA is issued last)
-    const int qb = LAST_FIRST ? QB - 1 - x / 4 : x / 4, gi = x % 4;
+    const int qb = x / 4, gi = x % 4;
     th.gm[qb][gi] = fmaxf(fmaxf(acc[qb][4 * gi], acc[qb][4 * gi + 1]),
                           fmaxf(acc[qb][4 * gi + 2], acc[qb][4 * gi + 3]));
   }
@@ -1273,22 +1270,12 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
     __syncthreads();                 // ... every wave's; and nobody still reads the other buffer
     const int buf = s & 1;
     const int t0s = c0 + s * G::SR;
+    if (s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
     const uint8_t* mb = smem + buf * G::MAIN_BYTES;
     const uint8_t* ab = smem + 2 * G::MAIN_BYTES + buf * G::AUG_BYTES;
     xg = rd(ab, AG);
 #pragma unroll
     for (int j = 0; j < (OVL ? J : PD); ++j) xa[j] = rdop(mb, A0 ^ (STEP * (uint32_t)j));
-    // the next stage's DMA: BF16 behind the stage's first MFMAs (they cover its issue), FP8 right
-    // after the first reads (measured: each the faster for its precision)
-    constexpr bool DMA_LATE = !F8;
-    if (!DMA_LATE && s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
-    if constexpr (!OVL) {   // the previous stage's last tile, while this stage's first reads land
-      if (s > 0) {
-#pragma unroll
-        for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS, true>(st, acc[0], th, p);
-        tile_filter<QB>(a, st, acc[0], th, prev_t0, prev_c1);
-      }
-    }
     // the norm step; without OVL it goes straight into block 0's accumulator (no register of its own)
     if constexpr (OVL) nrm = norm_step(xg);
     else acc[0][0] = norm_step(xg);
@@ -1314,7 +1301,6 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
           if (j + PD < J) xa[j + PD] = rdop(mb + rb * TILE_MAIN, A0 ^ (STEP * (uint32_t)(j + PD)));
           else if (more) xa[j + PD - J] = rdop(mn, A0 ^ (STEP * (uint32_t)(j + PD - J)));
         }
-        if (DMA_LATE && rb == 0 && j == 0 && s + 1 < nst) issue(t0s + G::SR, buf ^ 1);
         if (more && j == J / 2) {
           if constexpr (OVL) nrm = norm_step(xg);
           if constexpr (F8) sx_next = (int)(xg[2] & 0xFFu);
@@ -1327,13 +1313,11 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
         // the previous tile's appends (the last tile of the previous stage when rb = 0)
         if (rb > 0) tile_filter<QB>(a, st, acc[pv], th, (uint32_t)(t0s + 32 * (rb - 1)), c1);
         else tile_filter<QB>(a, st, acc[pv], th, prev_t0, prev_c1);
-      } else if (more) {
-        // (the stage's last tile is filtered after the next stage's barrier: its append stores
-        // then have a whole stage to complete before the wait for the next DMA)
+      } else {
 #pragma unroll
-        for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS, true>(st, acc[cur], th, p);
+        for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS>(st, acc[cur], th, p);
         tile_filter<QB>(a, st, acc[cur], th, (uint32_t)(t0s + 32 * rb), c1);
-        acc[0][0] = norm_step(xg);   // the next tile's norm, once this tile is filtered
+        if (more) acc[0][0] = norm_step(xg);   // the next tile's norm, once this tile is filtered
       }
       if (more) sx = sx_next;
     }
@@ -1350,10 +1334,6 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
 #pragma unroll
     for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS>(st, acc[1], th, p);
     tile_filter<QB>(a, st, acc[1], th, prev_t0, prev_c1);
-  } else if (nst > 0) {
-#pragma unroll
-    for (int p = 0; p < NPARTS; ++p) tile_max_part<QB, NPARTS, true>(st, acc[0], th, p);
-    tile_filter<QB>(a, st, acc[0], th, prev_t0, prev_c1);
   }
 #pragma unroll
   for (int qb = 0; qb < QB; ++qb) {
@@ -1381,26 +1361,6 @@ __device__ __forceinline__ int merge_rank(const MergeArgs& a, int K) {
 
 // New threshold of a merge whose K'-th entry is E: its key, or with the certificate band f(key),
 // never above the query's current threshold (which an in-scan truncation may have lowered).
-// merge_theta with its per-query inputs loaded ahead (the current threshold and, with the band,
-// the query's norm and rounding radius)
-struct ThetaIn {
-  uint32_t cur;
-  double qn2, qrho;
-};
-__device__ __forceinline__ ThetaIn theta_in(const MergeArgs& a, int q) {
-  ThetaIn t;
-  t.cur = a.thr[q];
-  t.qn2 = a.band.on ? a.band.qn2[q] : 0.0;
-  t.qrho = a.band.on ? a.band.qrho[q] : 0.0;
-  return t;
-}
-__device__ __forceinline__ uint32_t merge_theta_of(const MergeArgs& a, uint64_t E, const ThetaIn& in) {
-  const uint32_t tE = (uint32_t)(E >> 32);
-  if (!a.band.on) return tE;
-  const uint32_t f = band_thr_of(a.band, tE, in.qn2, in.qrho);
-  return f < in.cur ? f : (in.cur > tE ? in.cur : tE);
-}
-
 __device__ __forceinline__ uint32_t merge_theta(const MergeArgs& a, int q, uint64_t E) {
   const uint32_t tE = (uint32_t)(E >> 32);
   if (!a.band.on) return tE;
@@ -1778,7 +1738,6 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
   if (q >= (a.n_dev ? *a.n_dev : a.n)) return;   // wave-uniform; only wave-level sync below
   int* off = off_all[wv];
   uint16_t* map = map_all[wv];
-  const ThetaIn tin = theta_in(a, q);   // loaded now: used after the select
   const int nsrc = 1 + 2 * a.slices;
   const long subq = (long)q * a.slices_max * 2;
   const int P = (nsrc + 63) / 64;   // sources per lane (<= 5)
@@ -1901,7 +1860,7 @@ __global__ __launch_bounds__(256) void merge_reg_kernel(MergeArgs a) {
     }
     if (N > K) {
       E = wave_kth_regs<U>(v, N, K, kth_all[wv]);
-      theta = merge_theta_of(a, E, tin);
+      theta = merge_theta(a, q, E);
       if (banded) {
         int cnt = 0;
 #pragma unroll
