@@ -187,7 +187,7 @@ typedef struct {
   long max_batch;      /* query slots of the largest bucket (>= 1) */
   long mem_budget;     /* device bytes the server may hold; 0 = no limit. Buckets whose workspace
                           (times the lanes) does not fit are dropped at creation (the largest first) */
-  int lanes;           /* batches in flight at once, 1..3; 0 = 2 */
+  int lanes;           /* batches in flight at once, 1 or 2; 0 = 2 */
 } knns_server_config;
 
 typedef struct {

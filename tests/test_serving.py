@@ -28,7 +28,7 @@ def test_server_config_rejected_before_device_work(knns_mod):
     for prec, max_k, max_batch, budget, lanes, want in [
             (0, 0, 64, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 1025, 64, 0, 0, knns_mod.KNNS_ERR_INVALID),
             (0, 10, 0, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 10, 64, -1, 0, knns_mod.KNNS_ERR_INVALID),
-            (7, 10, 64, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 10, 64, 0, 4, knns_mod.KNNS_ERR_INVALID),
+            (7, 10, 64, 0, 0, knns_mod.KNNS_ERR_INVALID), (0, 10, 64, 0, 3, knns_mod.KNNS_ERR_INVALID),
             (0, 10, 64, 0, -1, knns_mod.KNNS_ERR_INVALID)]:
         cfg = knns_mod.ServerConfig(prec, max_k, max_batch, budget, lanes)
         assert L.knns_server_create(ctypes.byref(h), fake, ctypes.byref(cfg)) == want
