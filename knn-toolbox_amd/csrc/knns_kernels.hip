@@ -1297,7 +1297,7 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
         if constexpr (OVL) {
           if (more) xa[j] = rdop(mn, A0 ^ (STEP * (uint32_t)j));
         } else {
-          // PD k-steps ahead (2 QB MFMAs cover the read): PD + 1 operand sets live instead of J
+          // PD k-steps ahead (PD x QB MFMAs cover the read): PD + 1 operand sets live instead of J
           if (j + PD < J) xa[j + PD] = rdop(mb + rb * TILE_MAIN, A0 ^ (STEP * (uint32_t)(j + PD)));
           else if (more) xa[j + PD - J] = rdop(mn, A0 ^ (STEP * (uint32_t)(j + PD - J)));
         }
