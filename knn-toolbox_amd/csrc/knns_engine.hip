@@ -621,6 +621,7 @@ hipError_t enqueue_search(const SearchLaunch& L, const SearchBuffers& B, hipStre
     sa.n_dev = L.n_dev;
     sa.band = band;
     sa.thr_w = (uint32_t*)B.thr;
+    sa.wg_rows = P.groups / 4;
     // an overlapped upload: the scan reads up to one stage (<= 128 rows) past its round's end
     if (L.feed) KNNS_TRY(feed_rows(*L.feed, R.end + 128, st));
     if (hook) (*hook)(0);

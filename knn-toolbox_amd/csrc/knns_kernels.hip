@@ -1138,7 +1138,12 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar addressing
   const int col = lane & 31, h = lane >> 5;
-  const int wgrow = blockIdx.x, sl = blockIdx.y;   // workgroup row (4 query groups), slice
+  // workgroup row (4 query groups) and slice of this workgroup: the XCD-contiguous item of
+  // ScanArgs::wg_rows; the grid's padding to a multiple of 8 returns before any barrier
+  const int items = a.wg_rows * a.slices, per_xcd = (items + 7) >> 3;
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (item >= items) return;
+  const int wgrow = item % a.wg_rows, sl = item / a.wg_rows;
   if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
   const int g = wgrow * 4 + wv;
   const int c0 = (int)(a.row_begin + (long)sl * a.slice_rows);
@@ -2491,6 +2496,16 @@ static hipError_t scan_grid(void (*kern)(ScanArgs), const ScanArgs& a, int group
   return hipGetLastError();
 }
 
+// pipelined scans: one dimension, wg_rows x slices items rounded up to a multiple of 8
+// workgroups (the item mapping of ScanArgs::wg_rows)
+static hipError_t scan_grid1(void (*kern)(ScanArgs), const ScanArgs& a, int groups, hipStream_t stream) {
+  if (groups % 4 || a.wg_rows != groups / 4 || a.slices < 1) return hipErrorInvalidValue;
+  const long items = (long)a.wg_rows * a.slices;
+  if (items > 0x7fffff00L) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(kern, dim3((unsigned)((items + 7) / 8 * 8)), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
 template <int PREC, int QB, int DC, int SR>
 static hipError_t scan_one(const ScanArgs& a, int groups, hipStream_t stream) {
   return scan_grid(scan_kernel<PREC, QB, DC, SR>, a, groups, stream);
@@ -2539,7 +2554,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
     const int su = scan_units(prec, dc);
     const bool f8 = prec == PREC_FP8;
 #define KNNS_PIPE_CASE(B, S, F) \
-    if (qb == B && su == S && f8 == F) return scan_grid(scan_pipe_kernel<B, S, F>, a, groups, stream);
+    if (qb == B && su == S && f8 == F) return scan_grid1(scan_pipe_kernel<B, S, F>, a, groups, stream);
     KNNS_PIPE_CASE(1, 4, false) KNNS_PIPE_CASE(1, 8, false) KNNS_PIPE_CASE(1, 16, false)
     KNNS_PIPE_CASE(2, 4, false) KNNS_PIPE_CASE(2, 8, false) KNNS_PIPE_CASE(2, 16, false)
     KNNS_PIPE_CASE(1, 4, true) KNNS_PIPE_CASE(1, 8, true) KNNS_PIPE_CASE(1, 16, true)
