@@ -11,7 +11,7 @@ stream counts; here every knob shapes the one fused kernel sequence):
             resident workgroups per CU) everywhere, 3 and 4 for FP8 rows up to 128 bytes and BF16
             rows of 64 dims and 3 for BF16 at d = 128 (two per CU), 4 (one per CU) at d = 128;
   min_waves slices per query group: the smallest count that fills that many waves of resident
-            scan workgroups (the grid's second dimension);
+            scan workgroups (2 to 8);
   ratio     corpus growth between threshold rounds (0 = x4 while the prefix is below m/16, then x2);
   sl_extra  shortlist offset from the precision's built-in rule (BF16, FP32: exactness does not
             depend on it -- the certificate re-searches what it cannot prove -- only the cost does;
@@ -41,7 +41,7 @@ sys.path.insert(0, str(PKG))
 import knns  # noqa: E402
 
 QBS = (2, 1, 3, 4)             # the built-in choice for n > 32 first
-WAVES = (5, 3, 4, 6, 8)        # 5 = built-in
+WAVES = (5, 2, 3, 4, 6, 8)     # 5 = built-in
 RATIOS = (0, 2, 3, 4)          # 0 = built-in schedule
 EXTRAS = (0, -8, -16, -32, -48, 32)
 
