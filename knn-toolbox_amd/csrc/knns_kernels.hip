@@ -628,6 +628,17 @@ struct TileGeom {
 // loads of stage s+1 are in flight while the waves run the MFMAs of stage s. A BF16 stage of 128
 // rows is processed as two 64-row sub-tiles, halving barriers and waits per MFMA (single-chunk
 // only: a sub-tile's accumulators do not outlive the stage).
+// (workgroup row, slice) of this workgroup under the XCD-contiguous item order of
+// ScanArgs::wg_rows; false for the grid's padding to a multiple of 8 (the caller returns before
+// any barrier)
+__device__ __forceinline__ bool scan_item(const ScanArgs& a, int& wgrow, int& sl) {
+  const int items = a.wg_rows * a.slices, per_xcd = (items + 7) >> 3;
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  wgrow = item % a.wg_rows;
+  sl = item / a.wg_rows;
+  return item < items;
+}
+
 template <int PREC, int QB, int DC, int SR>
 __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   using G = TileGeom<PREC, DC, SR>;
@@ -638,8 +649,8 @@ __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int col = lane & 31, h = lane >> 5;
-  // grid: x = workgroup row (4 query groups), y = slice
-  const int wgrow = blockIdx.x, sl = blockIdx.y;
+  int wgrow, sl;   // workgroup row (4 query groups), slice
+  if (!scan_item(a, wgrow, sl)) return;
   // every query of the workgroup is an inert slot of a padded (graph-captured) search
   if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
   const int g = wgrow * 4 + wv;   // query group of this wave
@@ -866,7 +877,8 @@ __global__ __launch_bounds__(256, QB >= 3 ? 1 : 2) void scan_swz_kernel(ScanArgs
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar addressing
   const int col = lane & 31, h = lane >> 5;
-  const int wgrow = blockIdx.x, sl = blockIdx.y;   // workgroup row (4 query groups), slice
+  int wgrow, sl;   // workgroup row (4 query groups), slice
+  if (!scan_item(a, wgrow, sl)) return;
   // every query of the workgroup is an inert slot of a padded (graph-captured) search
   if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
   const int g = wgrow * 4 + wv;
@@ -1138,12 +1150,8 @@ __global__ __launch_bounds__(256, 2) void scan_pipe_kernel(ScanArgs a) {
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar addressing
   const int col = lane & 31, h = lane >> 5;
-  // workgroup row (4 query groups) and slice of this workgroup: the XCD-contiguous item of
-  // ScanArgs::wg_rows; the grid's padding to a multiple of 8 returns before any barrier
-  const int items = a.wg_rows * a.slices, per_xcd = (items + 7) >> 3;
-  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-  if (item >= items) return;
-  const int wgrow = item % a.wg_rows, sl = item / a.wg_rows;
+  int wgrow, sl;   // workgroup row (4 query groups), slice
+  if (!scan_item(a, wgrow, sl)) return;
   if (a.n_dev && wgrow * 4 * QB * 32 >= *a.n_dev) return;   // workgroup-uniform, before any barrier
   const int g = wgrow * 4 + wv;
   const int c0 = (int)(a.row_begin + (long)sl * a.slice_rows);
@@ -2490,15 +2498,10 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-static hipError_t scan_grid(void (*kern)(ScanArgs), const ScanArgs& a, int groups, hipStream_t stream) {
-  if (groups % 4) return hipErrorInvalidValue;  // a workgroup runs 4 query groups
-  hipLaunchKernelGGL(kern, dim3(groups / 4, a.slices), dim3(256), 0, stream, a);
-  return hipGetLastError();
-}
 
-// pipelined scans: one dimension, wg_rows x slices items rounded up to a multiple of 8
-// workgroups (the item mapping of ScanArgs::wg_rows)
-static hipError_t scan_grid1(void (*kern)(ScanArgs), const ScanArgs& a, int groups, hipStream_t stream) {
+// every scan: one dimension, wg_rows x slices items rounded up to a multiple of 8 workgroups (the
+// item order of ScanArgs::wg_rows)
+static hipError_t scan_grid(void (*kern)(ScanArgs), const ScanArgs& a, int groups, hipStream_t stream) {
   if (groups % 4 || a.wg_rows != groups / 4 || a.slices < 1) return hipErrorInvalidValue;
   const long items = (long)a.wg_rows * a.slices;
   if (items > 0x7fffff00L) return hipErrorInvalidValue;
@@ -2554,7 +2557,7 @@ hipError_t launch_scan(int prec, int qb, int dc, const ScanArgs& a, int groups, 
     const int su = scan_units(prec, dc);
     const bool f8 = prec == PREC_FP8;
 #define KNNS_PIPE_CASE(B, S, F) \
-    if (qb == B && su == S && f8 == F) return scan_grid1(scan_pipe_kernel<B, S, F>, a, groups, stream);
+    if (qb == B && su == S && f8 == F) return scan_grid(scan_pipe_kernel<B, S, F>, a, groups, stream);
     KNNS_PIPE_CASE(1, 4, false) KNNS_PIPE_CASE(1, 8, false) KNNS_PIPE_CASE(1, 16, false)
     KNNS_PIPE_CASE(2, 4, false) KNNS_PIPE_CASE(2, 8, false) KNNS_PIPE_CASE(2, 16, false)
     KNNS_PIPE_CASE(1, 4, true) KNNS_PIPE_CASE(1, 8, true) KNNS_PIPE_CASE(1, 16, true)
