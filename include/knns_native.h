@@ -84,7 +84,9 @@ typedef struct {
    * provably lies farther than its k-th exact distance (bounds: operand rounding radii and the
    * FP32 accumulation error of the approximate keys). Uncertified queries of the exact forms
    * (FP32, BF16, FP64) are searched again with every row inside the bound admitted, so their
-   * results are exact by construction; FP8 only counts them. */
+   * results are exact by construction (a BF16 query whose bound admits more rows than the
+   * re-search's 2048-row shortlist is searched once more on the FP32 rows); FP8 only counts
+   * them. */
   long uncertified;  /* queries whose first shortlist was not certified */
   long researched;   /* queries searched again */
   double research_s; /* device time of the re-search (with KNNS_KERNEL_TIMES, else 0) */
@@ -145,8 +147,8 @@ knns_status knns_search_noalloc(knns_corpus c, const void* queries, int dtype, l
  * (the whole-problem call of gpuknnsBitonic / gpuknnsHeap, src/cuknnBitonic.cu:422-618): host rows
  * (m x d of `dtype`) upload on a second stream while the first threshold rounds, which scan a
  * growing prefix of the corpus, already run. Host queries and outputs (KNNS_DEVICE_PTRS is
- * rejected); precision FP32 or FP64 (no low-precision corpus copy is built); results as
- * knns_search. */
+ * rejected); any precision (a BF16 or FP8 search packs its operand copy with the FP32 rows as
+ * they arrive); results as knns_search. */
 knns_status knns_search_once(const void* data, int dtype, long m, int d, const void* queries,
                              long n, const knns_params* p, void* dist, int32_t* idx);
 

@@ -19,8 +19,12 @@ constexpr int kDtype = KNNS_F64;
 constexpr int kPrec = KNNS_PREC_FP64;
 #else
 constexpr int kDtype = KNNS_F32;
-constexpr int kPrec = KNNS_PREC_FP32;
+// FP32 inputs are searched through the BF16 shortlist and the exact rerank: the shortlist
+// certificate (with its re-search, on the FP32 rows where BF16 cannot prove a query) makes the
+// answer the exact one, identical to the FP32 form's, at a ninth of its scan time
+constexpr int kPrec = KNNS_PREC_BF16;
 #endif
+constexpr unsigned kCorpusFlags = kPrec == KNNS_PREC_BF16 ? KNNS_CORPUS_BF16 : 0u;
 
 thread_local std::string g_err;
 thread_local knntimes g_times = {0.0, 0.0, 0.0};
@@ -141,7 +145,7 @@ double run_buckets(knntype* query, knntype* data, knntype* values, knntype* indi
   }
   (void)hipEventRecord(t0, nullptr);
   knns_corpus cp = nullptr;
-  knns_status st = knns_corpus_create(&cp, data, kDtype, rows, attributes, 0);
+  knns_status st = knns_corpus_create(&cp, data, kDtype, rows, attributes, kCorpusFlags);
   std::vector<int32_t> idx;
   for (int j = 0; j < c && st == KNNS_OK; ++j) {
     const long nq = query_sizes[j];

@@ -1096,7 +1096,17 @@ knns_status do_search(knns_corpus c, const void* queries, int dtype, long n, con
     EngineWorkspace ws(E);
     const size_t r_begin = ev++;
     if (ktimes) KNNS_HIP(hipEventRecord(E.event(r_begin), st));
-    if (knns_status rs = run_research(R, ws, st, E.cus, &researched)) return rs;
+    knns_status rs = run_research(R, ws, st, E.cus, &researched);
+    if (rs == KNNS_ERR_OVERFLOW && prec == KNNS_PREC_BF16) {
+      // more than the re-search's shortlist of rows lie within the BF16 error bound of some of
+      // these queries' k-th neighbour: search them once more on the FP32 operand rows (an error
+      // bound 2^16 times smaller) with every row admitted
+      g_err.clear();
+      R.prec = KNNS_PREC_FP32;
+      std::fill(R.thr.begin(), R.thr.end(), ORD_POS_INF);
+      rs = run_research(R, ws, st, E.cus, &researched);
+    }
+    if (rs) return rs;
     if (ktimes) {
       const size_t r_end = ev++;
       KNNS_HIP(hipEventRecord(E.event(r_end), st));
@@ -1528,7 +1538,10 @@ knns_status knns_search_once(const void* data, int dtype, long m, int d, const v
   if (!p) return fail(KNNS_ERR_INVALID, "null params");
   if (p->flags & KNNS_DEVICE_PTRS) return fail(KNNS_ERR_INVALID, "knns_search_once takes host buffers");
   knns_corpus c = nullptr;
-  if (knns_status st = corpus_alloc(&c, data, dtype, m, d, 0)) return st;
+  // the low-precision operand copy the search reads, packed with the FP32 rows as they arrive
+  const unsigned cflags = p->precision == KNNS_PREC_BF16 ? KNNS_CORPUS_BF16
+                          : (p->precision == KNNS_PREC_FP8 ? KNNS_CORPUS_FP8 : 0u);
+  if (knns_status st = corpus_alloc(&c, data, dtype, m, d, cflags)) return st;
   // the corpus goes up on the upload stream; each scan round waits only for the rows it reads
   Upload u;
   if (!E.ustream && hipStreamCreateWithFlags(&E.ustream, hipStreamNonBlocking) != hipSuccess) {

@@ -240,3 +240,29 @@ def test_easy_data_is_certified_in_one_pass(gpu, knns_mod, oracle_mod):
         Dg, Ig = c.search(Q, 16, precision="fp32")
         st = knns_mod.last_stats()
     assert st["uncertified"] == 0 and st["researched"] == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("call", ["dropin", "bf16"])
+def test_bf16_unprovable_queries_fall_back_to_fp32(gpu, knns_mod, oracle_mod, call):
+    """Rows 1 + 0.04 u: the BF16 rounding radius (up to 2^-8 per unit coordinate) is large against
+    the neighbour spacing, so the BF16 error bound of every query admits 7,000-8,000 of the 8192
+    rows (a NumPy model of the bound), more than the re-search's 2048-row shortlist; the FP32
+    bound admits 200-450. Those queries are searched once more on the FP32 rows and the answer is
+    the oracle's, through the drop-in call (which searches float inputs in BF16) and through a
+    resident BF16 search."""
+    g = np.random.default_rng(31)
+    m, d, n, k = 8192, 32, 16, 10
+    R = (1.0 + 0.04 * g.random((m, d))).astype(np.float32)
+    Q = (1.0 + 0.04 * g.random((n, d))).astype(np.float32)
+    D, I = oracle_mod.exact_topk(Q, R, k)
+    _case(f"bf16_fallback_{call}", m=m, d=d, n=n, k=k)
+    if call == "dropin":
+        vals, inds, _ = knns_mod.gpuknns_heap(Q, R, k)
+    else:
+        with knns_mod.Corpus(R, bf16=True) as c:
+            vals, inds = c.search(Q, k, precision="bf16")
+    st = knns_mod.last_stats()
+    assert st["researched"] == n
+    assert np.array_equal(inds.astype(np.int64), I.astype(np.int64))
+    assert np.array_equal(vals, D.astype(np.float32))
