@@ -284,7 +284,9 @@ def run_c3(args, knns, workloads, torch) -> dict:
 def run_c4(args, knns, workloads, torch) -> dict:
     """C4: 200k x 32 FP64 U[-1,1), all-kNN excluding self, k=8: the 64 log-uniform batches of
     the workload (sizes rescaled to sum to 200k; seed 7), issued in a shuffled order, packed into
-    one search by knns_search_batches (each batch excludes its own rows by index)."""
+    one search by knns_search_batches (each batch excludes its own rows by index). The double
+    corpus carries a BF16 copy: the shortlist is scanned in BF16, certified, and reranked exactly
+    from the FP64 rows (FP64 distances), as the double drop-in library searches."""
     X = workloads.workload_c4()
     m, d = X.shape
     k = 8
@@ -296,12 +298,12 @@ def run_c4(args, knns, workloads, torch) -> dict:
     rows = np.concatenate([np.arange(starts[b], starts[b] + sizes[b]) for b in order])
     bsz = np.ascontiguousarray(sizes[order], dtype=np.int64)
     bself = np.ascontiguousarray(starts[order], dtype=np.int64)
-    with knns.Corpus(X) as corpus:
+    with knns.Corpus(X, bf16=True) as corpus:
         dev = torch.device("cuda:0")
         Qd = torch.from_numpy(Qp).to(dev)
         Dd = torch.empty((m, k), dtype=torch.float64, device=dev)
         Id = torch.empty((m, k), dtype=torch.int32, device=dev)
-        p = corpus._params("fp64", k, "auto", 0, False, 0, 0, 0, knns.KNNS_DEVICE_PTRS)
+        p = corpus._params("bf16", k, "auto", 0, False, 0, 0, 0, knns.KNNS_DEVICE_PTRS)
 
         def run():
             st = knns.native().knns_search_batches(corpus._h, knns._ptr(Qd), knns.KNNS_F64, len(bsz),
@@ -320,6 +322,7 @@ def run_c4(args, knns, workloads, torch) -> dict:
             "data": "synthetic (U[-1,1), numpy PCG64 seed 6; batch sizes and order seed 7)",
             "config": {"workload": "C4: 200,000 x 32 FP64 all-kNN, 64 batches of 64..16384 rows in shuffled "
                                    "order, packed into one search (knns_search_batches)",
+                       "shortlist": "BF16 scan, certified, exact FP64 rerank from the FP64 rows",
                        "m": m, "d": d, "k": k, "batches": int(len(sizes)),
                        "batch_min": int(sizes.min()), "batch_max": int(sizes.max())},
             "ms_per_step": sec * 1e3, "sample_bit_exact_vs_fp64_oracle": ok, "sample_queries": int(len(rs))}
