@@ -69,7 +69,7 @@ struct ScanArgs {
   BandArgs band;         // in-scan compaction keeps the band above its K'-th entry; a truncation
                          // lowers thr[q] (atomicMin) so the merge never raises it past dropped rows
   uint32_t* thr_w;       // thr, writable (the same array)
-  // pipelined scans: items i = slice * wg_rows + row (i < G = wg_rows x slices) on a 1-D grid of
+  // scans: items i = slice * wg_rows + row (i < G = wg_rows x slices) on a 1-D grid of
   // round_up(G, 8) workgroups. The device hands workgroup b to XCD b % 8; workgroup b takes item
   // (b % 8) * ceil(G / 8) + b / 8, so that each XCD runs a contiguous run of items, every row of a
   // slice on one XCD (but at the run's ends), and fetches the slice's rows into its L2 once

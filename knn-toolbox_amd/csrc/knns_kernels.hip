@@ -623,11 +623,6 @@ struct TileGeom {
   static_assert(SR * SU % 64 == 0, "stage must be whole wave loads");
 };
 
-// One workgroup = 4 waves = 4 query groups (32*QB queries each) sharing one corpus slice. The
-// slice is streamed in stages of SR rows x one DC chunk through two LDS buffers filled by DMA: the
-// loads of stage s+1 are in flight while the waves run the MFMAs of stage s. A BF16 stage of 128
-// rows is processed as two 64-row sub-tiles, halving barriers and waits per MFMA (single-chunk
-// only: a sub-tile's accumulators do not outlive the stage).
 // (workgroup row, slice) of this workgroup under the XCD-contiguous item order of
 // ScanArgs::wg_rows; false for the grid's padding to a multiple of 8 (the caller returns before
 // any barrier)
@@ -639,6 +634,11 @@ __device__ __forceinline__ bool scan_item(const ScanArgs& a, int& wgrow, int& sl
   return item < items;
 }
 
+// One workgroup = 4 waves = 4 query groups (32*QB queries each) sharing one corpus slice. The
+// slice is streamed in stages of SR rows x one DC chunk through two LDS buffers filled by DMA: the
+// loads of stage s+1 are in flight while the waves run the MFMAs of stage s. A BF16 stage of 128
+// rows is processed as two 64-row sub-tiles, halving barriers and waits per MFMA (single-chunk
+// only: a sub-tile's accumulators do not outlive the stage).
 template <int PREC, int QB, int DC, int SR>
 __global__ __launch_bounds__(256, 2) void scan_kernel(ScanArgs a) {
   using G = TileGeom<PREC, DC, SR>;
@@ -2497,7 +2497,6 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL(pack_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
-
 
 // every scan: one dimension, wg_rows x slices items rounded up to a multiple of 8 workgroups (the
 // item order of ScanArgs::wg_rows)
